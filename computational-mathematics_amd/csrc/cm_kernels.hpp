@@ -1,0 +1,420 @@
+// Device kernels of the preconditioned BiCGSTAB solve engine (gfx950).
+//
+// Every operation on the solve path is HBM-bandwidth bound (sparse
+// matrix-vector products and level-1 vector work); nothing is a dense
+// contraction, so there is no MFMA here.  Design points:
+//  * CSR SpMV, CSR-Adaptive style: the host cuts the rows into blocks of
+//    consecutive rows holding at most TILE nonzeros (or one long row).  A
+//    256-thread workgroup streams its block's (value, column) pairs with fully
+//    coalesced loads, forms value*x[column] into LDS, then reduces each row
+//    with a power-of-two group of lanes (1..64) chosen from the block's row
+//    count, finishing with cross-lane xor shuffles inside the wave.  A single
+//    row longer than TILE is reduced by the whole workgroup.  Every nonzero
+//    and every column index is read exactly once.
+//  * Epilogues are fused into the SpMV row write: D^{-1} scaling (factor
+//    apply), b - A x (residuals), beta = 1 accumulation (Richardson sweeps),
+//    and the dot products BiCGSTAB needs right after A-products.
+//  * Reductions are deterministic: each workgroup writes its partial sums, the
+//    last workgroup to arrive (agent-scope acq_rel counter) adds the partials
+//    in a fixed order and updates the solve state.  No float atomics.
+//  * The Krylov scalars and the exit decision live in device memory
+//    (SolveState), so the host can enqueue many iterations without reading
+//    anything back; every kernel returns at once when the state says done.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <hip/hip_fp16.h>
+
+#include <cstdint>
+
+namespace cm {
+
+constexpr int BS = 256;       // threads per workgroup (4 waves of 64)
+constexpr int TILE = 2048;    // nonzeros staged per SpMV workgroup
+constexpr int ROWCAP = 1024;  // rows per SpMV workgroup
+constexpr int WAVE = 64;
+constexpr int MAX_RED_BLOCKS = 2048;  // workgroups of the reducing vector kernels
+
+// Solve state kept in device memory.  variant: 0 AFIIM, 1 PDFIIM, 2 ACIFIM.
+struct SolveState {
+    double ro, rc, prc, m1, m2, alpha, omega, beta;
+    double tol, eps;
+    int iter, nmax, stag, msteps, maxmsteps, info, done, variant;
+    int ntrace, trace_cap;
+    unsigned int arrive[4];  // arrival counters of the last-block reductions
+    double *trace;           // device array of trace_cap doubles or null
+};
+
+// ---------------------------------------------------------------- reductions
+
+template <class T>
+__device__ __forceinline__ T wave_sum(T v)
+{
+#pragma unroll
+    for (int off = WAVE / 2; off > 0; off >>= 1) v += __shfl_xor(v, off, WAVE);
+    return v;
+}
+
+// Sum of K values over the workgroup; result valid on thread 0.
+template <int K>
+__device__ __forceinline__ void block_sum(double (&v)[K], double *lds /* K*BS/WAVE */)
+{
+    const int lane = threadIdx.x & (WAVE - 1), w = threadIdx.x / WAVE;
+#pragma unroll
+    for (int k = 0; k < K; k++) v[k] = wave_sum(v[k]);
+    __syncthreads();
+    if (lane == 0)
+#pragma unroll
+        for (int k = 0; k < K; k++) lds[w * K + k] = v[k];
+    __syncthreads();
+    if (threadIdx.x == 0)
+#pragma unroll
+        for (int k = 0; k < K; k++) {
+            double s = 0.0;
+            for (int q = 0; q < BS / WAVE; q++) s += lds[q * K + k];
+            v[k] = s;
+        }
+}
+
+// Writes this workgroup's K partials; returns true (on every thread) in the
+// last workgroup to arrive, which then holds the grid totals on thread 0.
+template <int K>
+__device__ __forceinline__ bool grid_sum_last(double (&v)[K], double *partials,
+                                              unsigned int *arrive, double *lds)
+{
+    __shared__ int am_last;
+    block_sum<K>(v, lds);
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int k = 0; k < K; k++)
+            __hip_atomic_store(&partials[(size_t)blockIdx.x * K + k], v[k], __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_AGENT);
+        __threadfence();
+        unsigned int prev = __hip_atomic_fetch_add(arrive, 1u, __ATOMIC_ACQ_REL,
+                                                   __HIP_MEMORY_SCOPE_AGENT);
+        am_last = (prev == gridDim.x - 1);
+    }
+    __syncthreads();
+    if (!am_last) return false;
+    __threadfence();
+    double s[K];
+#pragma unroll
+    for (int k = 0; k < K; k++) s[k] = 0.0;
+    for (int b = threadIdx.x; b < (int)gridDim.x; b += BS)
+#pragma unroll
+        for (int k = 0; k < K; k++)
+            s[k] += __hip_atomic_load(&partials[(size_t)b * K + k], __ATOMIC_RELAXED,
+                                      __HIP_MEMORY_SCOPE_AGENT);
+    block_sum<K>(s, lds);
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int k = 0; k < K; k++) v[k] = s[k];
+        __hip_atomic_store(arrive, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    return true;
+}
+
+// ------------------------------------------------------------ SpMV epilogues
+//
+// An epilogue sees each row's sum once (row, value) and may accumulate K
+// per-row products into the workgroup's reduction; fin() runs on thread 0 of
+// the last workgroup with the grid totals.
+
+template <class Y>
+struct EpiStore {  // y = A x
+    static constexpr int K = 0;
+    Y *y;
+    __device__ void row(int r, double s, double *) const { y[r] = (Y)s; }
+    __device__ void fin(SolveState *, const double *) const {}
+};
+
+template <class Y, class D>
+struct EpiScale {  // y = d .* (A x)   (factor apply: D^{-1} after H)
+    static constexpr int K = 0;
+    Y *y;
+    const D *d;
+    __device__ void row(int r, double s, double *) const { y[r] = (Y)((double)d[r] * (double)(Y)s); }
+    __device__ void fin(SolveState *, const double *) const {}
+};
+
+template <class Y>
+struct EpiSubFrom {  // y = b - A x
+    static constexpr int K = 0;
+    Y *y;
+    const Y *b;
+    __device__ void row(int r, double s, double *) const { y[r] = (Y)((double)b[r] - (double)(Y)s); }
+    __device__ void fin(SolveState *, const double *) const {}
+};
+
+template <class Y>
+struct EpiAccum {  // y = A x + y
+    static constexpr int K = 0;
+    Y *y;
+    __device__ void row(int r, double s, double *) const { y[r] = (Y)((double)(Y)s + (double)y[r]); }
+    __device__ void fin(SolveState *, const double *) const {}
+};
+
+template <class Y, class D>
+struct EpiAccumScale {  // z = (A x + y) .* d   (last Richardson sweep then D^{-1})
+    static constexpr int K = 0;
+    const Y *y;
+    Y *z;
+    const D *d;
+    __device__ void row(int r, double s, double *) const
+    {
+        Y t = (Y)((double)(Y)s + (double)y[r]);
+        z[r] = (Y)((double)t * (double)d[r]);
+    }
+    __device__ void fin(SolveState *, const double *) const {}
+};
+
+// Setup: r = b - A x; r0 = p = r; ro = ||r||, m1 = ro^2
+// (AFIIM/src/bicgstab.cpp:149-165).
+struct EpiSetup {
+    static constexpr int K = 1;
+    double *r, *r0, *p;
+    const double *b;
+    __device__ void row(int i, double s, double *acc) const
+    {
+        const double v = b[i] + (-1.0) * s;
+        r[i] = v;
+        r0[i] = v;
+        p[i] = v;
+        acc[0] += v * v;
+    }
+    __device__ void fin(SolveState *st, const double *tot) const
+    {
+        st->ro = sqrt(tot[0]);
+        st->m1 = st->ro * st->ro;
+        st->rc = st->ro;
+        st->iter = 0;
+        st->stag = 0;
+        st->msteps = 0;
+        st->ntrace = 0;
+        if (st->nmax <= 0) {
+            st->done = 1;
+            if (st->nmax == 0) st->info = 3;
+        }
+    }
+};
+
+// v = A pc, alpha = m1 / (r0 . v)   (AFIIM/src/bicgstab.cpp:177-181;
+// ACIFIM/src/bicgstab.cpp:205-213 adds the non-finite guard).
+struct EpiAlpha {
+    static constexpr int K = 1;
+    double *v;
+    const double *r0;
+    __device__ void row(int i, double s, double *acc) const
+    {
+        v[i] = s;
+        acc[0] += r0[i] * s;
+    }
+    __device__ void fin(SolveState *st, const double *tot) const
+    {
+        const double a = st->m1 / tot[0];
+        st->alpha = a;
+        if (st->variant == 2 && !isfinite(a)) {
+            st->iter = st->nmax;
+            st->info = 3;
+            st->done = 1;
+        }
+    }
+};
+
+// t = A sc, omega = (t . s) / (t . t)   (AFIIM/src/bicgstab.cpp:193-198).
+struct EpiOmega {
+    static constexpr int K = 2;
+    double *t;
+    const double *s;
+    __device__ void row(int i, double v, double *acc) const
+    {
+        t[i] = v;
+        acc[0] += v * s[i];
+        acc[1] += v * v;
+    }
+    __device__ void fin(SolveState *st, const double *tot) const { st->omega = tot[0] / tot[1]; }
+};
+
+// ------------------------------------------------------------------- SpMV
+
+template <class V>
+__device__ __forceinline__ double ldv(const V *p, int k) { return (double)p[k]; }
+template <>
+__device__ __forceinline__ double ldv<__half>(const __half *p, int k) { return (double)__half2float(p[k]); }
+
+// y = A x (+ epilogue).  A: CSR ptr/col/val; blk: row-block starts (nblk + 1).
+// Accumulation is in double for every value/vector type (the FP64 path and
+// the FP32 inner solve share the kernel; FP32 inputs are exact in double).
+template <class V, class X, class Epi>
+__global__ __launch_bounds__(BS) void spmv_kernel(const int *__restrict__ ptr,
+                                                  const int *__restrict__ col,
+                                                  const V *__restrict__ val,
+                                                  const float *__restrict__ rowscale,
+                                                  const int *__restrict__ blk,
+                                                  const X *__restrict__ x, Epi epi,
+                                                  SolveState *st, double *partials)
+{
+    if (st && st->done) return;
+    __shared__ double prod[TILE];
+    __shared__ double red[(Epi::K > 0 ? Epi::K : 1) * BS / WAVE];
+    double acc[Epi::K > 0 ? Epi::K : 1];
+#pragma unroll
+    for (int k = 0; k < (Epi::K > 0 ? Epi::K : 1); k++) acc[k] = 0.0;
+
+    const int tid = threadIdx.x;
+    const int r0 = blk[blockIdx.x], r1 = blk[blockIdx.x + 1];
+    const int rs = ptr[r0], re = ptr[r1];
+    const int cnt = re - rs;
+    const int nr = r1 - r0;
+
+    if (cnt > TILE) {
+        // one long row: the whole workgroup strides over it
+        double s = 0.0;
+        for (int k = rs + tid; k < re; k += BS) s += ldv(val, k) * (double)x[col[k]];
+        double v1[1] = {s};
+        block_sum<1>(v1, prod);
+        if (tid == 0) {
+            double rsum = v1[0];
+            if (rowscale) rsum *= (double)rowscale[r0];
+            epi.row(r0, rsum, acc);
+        }
+    } else {
+        int cidx[TILE / BS];
+        double cv[TILE / BS];
+#pragma unroll
+        for (int u = 0; u < TILE / BS; u++) {
+            const int k = tid + u * BS;
+            if (k < cnt) {
+                cidx[u] = col[rs + k];
+                cv[u] = ldv(val, rs + k);
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < TILE / BS; u++) {
+            const int k = tid + u * BS;
+            if (k < cnt) prod[k] = cv[u] * (double)x[cidx[u]];
+        }
+        __syncthreads();
+        int tpr = 1;  // lanes per row: a power of two, at most one wave
+        while (tpr < WAVE && tpr * 2 * nr <= BS) tpr *= 2;
+        const int groups = BS / tpr;
+        const int g = tid / tpr, lane = tid & (tpr - 1);
+        for (int base = 0; base < nr; base += groups) {
+            const int rr = base + g;
+            const bool live = rr < nr;
+            double s = 0.0;
+            if (live) {
+                const int a = ptr[r0 + rr] - rs, e = ptr[r0 + rr + 1] - rs;
+                for (int k = a + lane; k < e; k += tpr) s += prod[k];
+            }
+            for (int off = tpr / 2; off > 0; off >>= 1) s += __shfl_xor(s, off, WAVE);
+            if (live && lane == 0) {
+                if (rowscale) s *= (double)rowscale[r0 + rr];
+                epi.row(r0 + rr, s, acc);
+            }
+        }
+    }
+    if constexpr (Epi::K > 0) {
+        if (grid_sum_last<Epi::K>(acc, partials, &st->arrive[0], red) && tid == 0) epi.fin(st, acc);
+    }
+}
+
+// ----------------------------------------------------------- vector kernels
+
+// s = r - alpha v   (AFIIM/src/bicgstab.cpp:183-185)
+__global__ __launch_bounds__(BS) void k_s_update(int n, const double *__restrict__ r,
+                                                 const double *__restrict__ v,
+                                                 double *__restrict__ s, const SolveState *st)
+{
+    if (st->done) return;
+    const double na = -st->alpha;
+    for (int i = blockIdx.x * BS + threadIdx.x; i < n; i += gridDim.x * BS) s[i] = r[i] + na * v[i];
+}
+
+// One pass over x, pc, sc, r, s, t, r0:
+//   ||pc||^2, ||x_old||^2, ||r_old||^2;  x += alpha pc;  x += omega sc;
+//   r = s - omega t;  r0 . r_new.
+// The last workgroup applies the exit tests in the reference's order
+// (divergence, stagnation, convergence on the lagged ||r||) and, when the
+// loop goes on, forms beta (AFIIM/src/bicgstab.cpp:200-274).
+__global__ __launch_bounds__(BS) void k_xr_update(int n, double *__restrict__ x,
+                                                  const double *__restrict__ pc,
+                                                  const double *__restrict__ sc,
+                                                  double *__restrict__ r,
+                                                  const double *__restrict__ s,
+                                                  const double *__restrict__ t,
+                                                  const double *__restrict__ r0,
+                                                  SolveState *st, double *partials)
+{
+    if (st->done) return;
+    __shared__ double red[4 * BS / WAVE];
+    const double a = st->alpha, om = st->omega, nom = -om;
+    double acc[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int i = blockIdx.x * BS + threadIdx.x; i < n; i += gridDim.x * BS) {
+        const double pci = pc[i], xi = x[i], ri = r[i];
+        acc[0] += pci * pci;
+        acc[1] += xi * xi;
+        acc[2] += ri * ri;
+        double xn = xi + a * pci;
+        xn = xn + om * sc[i];
+        x[i] = xn;
+        const double rn = s[i] + nom * t[i];
+        r[i] = rn;
+        acc[3] += r0[i] * rn;
+    }
+    if (grid_sum_last<4>(acc, partials, &st->arrive[1], red) && threadIdx.x == 0) {
+        const double pcn = sqrt(acc[0]), xnrm = sqrt(acc[1]);
+        if (fabs(st->alpha) * pcn < st->eps * xnrm) st->stag++;
+        else st->stag = 0;
+        st->prc = st->rc;
+        st->rc = sqrt(acc[2]);
+        if (st->trace && st->ntrace < st->trace_cap) st->trace[st->ntrace] = st->rc;
+        st->ntrace++;
+        if (st->prc < st->rc) st->msteps++;
+        else st->msteps = 0;
+        if (st->msteps >= st->maxmsteps) {
+            st->info = 1;
+            st->done = 1;
+        } else if (st->stag >= 3) {
+            st->info = 2;
+            st->done = 1;
+        } else if (st->rc < st->tol * st->ro) {
+            st->info = 0;
+            st->done = 1;
+        } else {
+            st->m2 = st->m1;
+            st->m1 = acc[3];
+            st->beta = st->m1 * st->alpha / (st->omega * st->m2);
+            st->iter++;
+            if (st->iter >= st->nmax) {
+                st->info = 3;
+                st->done = 1;
+            }
+        }
+    }
+}
+
+// p = (-beta omega) v + beta p;  p = p + r   (AFIIM/src/bicgstab.cpp:269-271)
+__global__ __launch_bounds__(BS) void k_p_update(int n, double *__restrict__ p,
+                                                 const double *__restrict__ r,
+                                                 const double *__restrict__ v, const SolveState *st)
+{
+    // runs after k_xr_update of the same pass; when that pass exited, p is dead
+    if (st->done) return;
+    const double be = st->beta, c = -st->beta * st->omega;
+    for (int i = blockIdx.x * BS + threadIdx.x; i < n; i += gridDim.x * BS) {
+        const double q = c * v[i] + be * p[i];
+        p[i] = q + r[i];
+    }
+}
+
+// z = a .* b
+template <class T, class D>
+__global__ __launch_bounds__(BS) void k_vmul(int n, const T *__restrict__ a, const D *__restrict__ d,
+                                             T *__restrict__ z, const SolveState *st)
+{
+    if (st && st->done) return;
+    for (int i = blockIdx.x * BS + threadIdx.x; i < n; i += gridDim.x * BS)
+        z[i] = (T)((double)a[i] * (double)d[i]);
+}
+
+}  // namespace cm

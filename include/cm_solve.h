@@ -1,0 +1,104 @@
+/*
+ * cm_solve.h -- C ABI of the MI355X BiCGSTAB solve engine (libcmsolve.so).
+ *
+ * The three drop-in entry points take the same arguments, in the same order
+ * and with the same meaning, as the reference solve functions they replace,
+ * and add an int status (CM_OK or a negative CM_ERR_*).  They read A, b and
+ * the factors, write only x (in/out), *resval, *iter and *info, and keep no
+ * pointer past return.  info: 0 converged, 1 diverged, 2 stagnated, 3 NMAX
+ * reached or non-finite alpha (ACIFIM).  The C++ overloads with the
+ * reference's exact names live in cm_dropin.hpp.
+ */
+#ifndef CM_SOLVE_H
+#define CM_SOLVE_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define CM_OK 0
+#define CM_ERR_HIP (-1)          /* HIP runtime error (message: cm_last_error) */
+#define CM_ERR_ARG (-2)          /* invalid argument */
+#define CM_ERR_UNSUPPORTED (-3)  /* option not available in this build */
+#define CM_ERR_NODEV (-4)        /* no usable GPU */
+
+/* Replaces AFIIM EPBiCGSTAB (AFIIM/src/bicgstab.hpp:82-100,
+ * AFIIM/src/bicgstab.cpp:87-297): M = G diag(IDv) H, explicit apply. */
+int cm_afiim_epbicgstab(int n, double *Av, int *Aj, int *Ai, double *b, double tol,
+                        int NMAX, double *Gv, int *Gj, int *Gi, double *IDv,
+                        double *Hv, int *Hj, int *Hi, double *x, double *resval,
+                        int *iter, int verbose, int *info);
+
+/* Replaces PDFIIM EPBiCGSTAB (PDFIIM/src/bicgstab.hpp:75-86,
+ * PDFIIM/src/bicgstab.cpp:80-310), with sparseMatrix<int,int,double>
+ * flattened to its CSR arrays.  a_symmetric != 0 means A holds only the upper
+ * triangle of a symmetric matrix (mtype SYMMETRIC) and M = G D^{-1} G^T; H is
+ * then not read and may be NULL. */
+int cm_pdfiim_epbicgstab(int n, int a_symmetric, const int *Ai, const int *Aj,
+                         const double *Av, const double *b, double tol, int NMAX,
+                         const int *Gi, const int *Gj, const double *Gv,
+                         const double *IDv, const int *Hi, const int *Hj,
+                         const double *Hv, double *x, double *resval, int *iter,
+                         int verbose, int *info);
+
+/* Replaces ACIFIM PBiCGSTAB (ACIFIM/src/bicgstab.hpp:94-120,
+ * ACIFIM/src/bicgstab.cpp:97-327) with apply() modes 'E' (explicit
+ * G D^{-1} H), 'C' (nr Richardson sweeps with L and U), 'I' (L, U triangular
+ * solves with D^{-1}) and 'J' (L, U solves) (ACIFIM/src/bicgstab.cpp:372-428). */
+int cm_acifim_pbicgstab(int n, double *Av, int *Aj, int *Ai, double *b, double tol,
+                        int NMAX, double *Gv, int *Gj, int *Gi, double *IDv,
+                        double *Hv, int *Hj, int *Hi, double *Lv, int *Lj, int *Li,
+                        double *Uv, int *Uj, int *Ui, int nr, char mode, double *x,
+                        double *resval, int *iter, int verbose, int *info);
+
+/* ---- resident systems (A and factors kept in HBM across solves) ---- */
+
+typedef struct cm_system_s *cm_system;
+
+#define CM_VARIANT_AFIIM 0
+#define CM_VARIANT_PDFIIM 1
+#define CM_VARIANT_ACIFIM 2
+
+/* Uploads A and the factors once.  variant selects the reference loop whose
+ * constants and exit rules apply; L/U (ACIFIM modes 'C','I','J') may be NULL
+ * otherwise; H may be NULL when a_symmetric (PDFIIM). */
+int cm_system_create(cm_system *sys, int variant, int n, int a_symmetric,
+                     const int *Ai, const int *Aj, const double *Av,
+                     const int *Gi, const int *Gj, const double *Gv,
+                     const double *IDv, const int *Hi, const int *Hj,
+                     const double *Hv, const int *Li, const int *Lj,
+                     const double *Lv, const int *Ui, const int *Uj,
+                     const double *Uv, int nr, char mode);
+int cm_system_destroy(cm_system sys);
+
+/* Device-resident right-hand side and solution of the system (n doubles each):
+ * set_rhs uploads b; set_x uploads x (or zeroes it when x == NULL); get_x
+ * downloads x. */
+int cm_system_set_rhs(cm_system sys, const double *b);
+int cm_system_set_x(cm_system sys, const double *x);
+int cm_system_get_x(cm_system sys, double *x);
+
+/* FP64 solve on the resident b and x (x holds the initial guess).
+ * trace (optional, host, trace_cap doubles) receives ||r_k|| of every pass
+ * (the lagged residual the reference tests, AFIIM/src/bicgstab.cpp:213-216);
+ * *ntrace = passes that computed it.  info is in/out as in the reference (it
+ * is left unchanged when NMAX < 0). */
+int cm_system_solve(cm_system sys, double tol, int NMAX, double *resval, int *iter,
+                    int *info, double *trace, int trace_cap, int *ntrace);
+
+/* Kernel-level entry points (host vectors in/out), for parity checks:
+ * which = 'A', 'G', 'H', 'L', 'U' (and 'T' for the G^T copy of a symmetric
+ * PDFIIM system): y = M x.  apply: out = M v with the system's preconditioner. */
+int cm_system_spmv(cm_system sys, char which, const double *x, double *y);
+int cm_system_apply(cm_system sys, const double *v, double *out);
+
+/* Stored sizes (after symmetric expansion / transposition): which as above. */
+long long cm_system_nnz(cm_system sys, char which);
+
+const char *cm_last_error(void);
+int cm_device_count(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,162 @@
+"""GPU parity of the FP64 solve path against the oracle (CPU restatement of the
+reference solve loops), through the C ABI.
+
+Bars: SpMV and M-apply within 1e-12 relative (normwise) of the oracle; the
+per-pass residual norms within 1e-8 relative over the first 20 passes;
+iteration counts within 2%; identical info codes; the true relative residual
+of the returned x meets the tolerance.
+"""
+import numpy as np
+import pytest
+
+import cmamd
+import oracle
+
+pytestmark = pytest.mark.gpu
+
+SPMV_TOL = 1e-12
+TRACE_TOL = 1e-8
+
+
+def _rel(a, b):
+    return np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300)
+
+
+def _system(kind):
+    if kind == "c1":
+        return cmamd.gen_convdiff(2, 100, 0.5, seed=1)
+    if kind == "cd3":
+        return cmamd.gen_convdiff(3, 20, 1.0, seed=5)
+    if kind == "irregular":
+        return cmamd.gen_stencil27(14, 0.05, 200, 600, seed=9)
+    raise KeyError(kind)
+
+
+def _random_csr(n, seed, long_rows=(), empty_rows=()):
+    """Random CSR with a dominant diagonal, a few very long rows (longer than
+    one SpMV tile) and some empty rows."""
+    import scipy.sparse as sp
+    rng = np.random.default_rng(seed)
+    rows, cols = [], []
+    for i in range(n):
+        if i in empty_rows:
+            continue
+        k = 3000 if i in long_rows else rng.integers(1, 12)
+        c = rng.choice(n, size=min(k, n), replace=False)
+        rows += [i] * len(c)
+        cols += list(c)
+    m = sp.csr_matrix((rng.uniform(-1, 1, len(rows)), (rows, cols)), shape=(n, n))
+    m.sum_duplicates()
+    return cmamd.Csr.from_scipy(m)
+
+
+@pytest.mark.parametrize("kind", ["c1", "cd3", "irregular"])
+def test_spmv_matches_oracle(kind):
+    A = _system(kind)
+    F = cmamd.afiim(A, 0.1)
+    sys_ = cmamd.System("afiim", A, F)
+    x = cmamd.gen_uniform(3, A.n)
+    for which, M in (("A", A), ("G", F.G), ("H", F.H)):
+        y = sys_.spmv(which, x)
+        assert _rel(y, oracle.csr_mv(M, x)) <= SPMV_TOL, which
+    sys_.close()
+
+
+def test_spmv_long_and_empty_rows():
+    A = _random_csr(5000, 3, long_rows={7, 4000, 4999}, empty_rows={0, 1, 2500})
+    # identity factors: the system only needs a G and an H
+    I = cmamd.Csr(A.n, np.arange(A.n + 1, dtype=np.int32), np.arange(A.n, dtype=np.int32), np.ones(A.n))
+    F = cmamd.Factors(I, np.ones(A.n), I)
+    sys_ = cmamd.System("afiim", A, F)
+    x = np.random.default_rng(1).uniform(-1, 1, A.n)
+    y = sys_.spmv("A", x)
+    ref = oracle.csr_mv(A, x)
+    assert _rel(y, ref) <= SPMV_TOL
+    assert np.all(y[[0, 1, 2500]] == 0.0)
+    sys_.close()
+
+
+@pytest.mark.parametrize("kind", ["c1", "cd3"])
+def test_apply_matches_oracle(kind):
+    A = _system(kind)
+    F = cmamd.afiim(A, 0.1)
+    sys_ = cmamd.System("afiim", A, F)
+    v = cmamd.gen_uniform(4, A.n)
+    assert _rel(sys_.apply(v), oracle.apply(F, v, mode="E")) <= SPMV_TOL
+    sys_.close()
+
+
+def _compare_solve(variant, A, F, b, nmax, tol=1e-8, **kw):
+    sys_ = cmamd.System(variant, A, F, **kw)
+    sys_.set_rhs(b)
+    sys_.set_x(None)
+    rv, it, info, tr = sys_.solve(tol=tol, nmax=nmax, trace_cap=64)
+    x = sys_.get_x()
+    sys_.close()
+    okw = {k: v for k, v in kw.items() if k in ("nr", "mode")}
+    xo, rvo, ito, info_o, tro = oracle.bicgstab(variant, A, b, F, tol=tol, nmax=nmax, trace_cap=64, **okw)
+    m = min(20, len(tr), len(tro))
+    assert m > 0
+    assert np.max(np.abs(tr[:m] - tro[:m]) / tro[:m]) <= TRACE_TOL
+    assert info == info_o
+    assert abs(it - ito) <= max(1, int(np.ceil(0.02 * ito)))
+    return x, rv, it, info, xo
+
+
+def test_solve_c1_afiim():
+    A = _system("c1")
+    xs = cmamd.gen_uniform(2, A.n)
+    b = cmamd.csr_mv_host(A, xs)
+    F = cmamd.afiim(A, 0.1)
+    x, rv, it, info, xo = _compare_solve("afiim", A, F, b, 500)
+    assert info == 0
+    assert np.linalg.norm(b - cmamd.csr_mv_host(A, x)) / np.linalg.norm(b) <= 1e-7
+    assert _rel(x, xo) <= 1e-6
+
+
+def test_solve_3d_pdfiim():
+    A = _system("cd3")
+    xs = cmamd.gen_uniform(6, A.n)
+    b = cmamd.csr_mv_host(A, xs)
+    F = cmamd.pdfiim(A, 2, 2, 1, 0.05, -1, "I", "B")
+    x, rv, it, info, xo = _compare_solve("pdfiim", A, F, b, 1200)
+    assert info == 0
+
+
+@pytest.mark.parametrize("mode,nr", [("E", 1), ("C", 2), ("C", 3)])
+def test_solve_3d_acifim(mode, nr):
+    A = cmamd.gen_convdiff(3, 16, 5.0, seed=8)
+    xs = cmamd.gen_uniform(7, A.n)
+    b = cmamd.csr_mv_host(A, xs)
+    F = cmamd.acifim(A, 0.01, 0.1)
+    sys_ = cmamd.System("acifim", A, F, nr=nr, mode=mode)
+    v = cmamd.gen_uniform(8, A.n)
+    assert _rel(sys_.apply(v), oracle.apply(F, v, nr=nr, mode=mode)) <= SPMV_TOL
+    sys_.close()
+    _compare_solve("acifim", A, F, b, 2000, nr=nr, mode=mode)
+
+
+def test_forced_exits():
+    A = _system("c1")
+    xs = cmamd.gen_uniform(2, A.n)
+    b = cmamd.csr_mv_host(A, xs)
+    F = cmamd.afiim(A, 0.1)
+    # NMAX reached
+    x, rv, it, info, _ = _compare_solve("afiim", A, F, b, 2)
+    assert (it, info) == (2, 3)
+    # tolerance below rounding: the loop must end by stagnation or divergence
+    # exactly where the oracle's does
+    x, rv, it, info, _ = _compare_solve("afiim", A, F, b, 500, tol=1e-30)
+    assert info in (1, 2)
+
+
+def test_dropin_entry_points():
+    A = _system("c1")
+    xs = cmamd.gen_uniform(2, A.n)
+    b = cmamd.csr_mv_host(A, xs)
+    F = cmamd.afiim(A, 0.1)
+    r = cmamd.afiim_epbicgstab(A, b, F, nmax=500)
+    xo, rvo, ito, info_o, _ = oracle.bicgstab("afiim", A, b, F, nmax=500)
+    assert r.info == info_o == 0
+    assert abs(r.iter - ito) <= 1
+    assert _rel(r.x, xo) <= 1e-6

@@ -276,7 +276,8 @@ def csr_mv_host(A: Csr, x):
 SOLVE_SYMBOLS = [
     "cm_afiim_epbicgstab", "cm_pdfiim_epbicgstab", "cm_acifim_pbicgstab", "cm_system_create",
     "cm_system_destroy", "cm_system_set_rhs", "cm_system_set_x", "cm_system_get_x", "cm_system_solve",
-    "cm_system_spmv", "cm_system_apply", "cm_system_nnz", "cm_last_error", "cm_device_count",
+    "cm_system_spmv", "cm_system_apply", "cm_system_nnz", "cm_system_profile", "cm_system_kernel_stats",
+    "cm_last_error", "cm_device_count",
 ]
 
 
@@ -305,6 +306,10 @@ def _type_solve(lib):
     lib.cm_system_apply.argtypes = [vp, PD, PD]
     lib.cm_system_nnz.argtypes = [vp, C.c_char]
     lib.cm_system_nnz.restype = C.c_longlong
+    lib.cm_system_profile.argtypes = [vp, C.c_int]
+    lib.cm_system_profile.restype = C.c_int
+    lib.cm_system_kernel_stats.argtypes = [vp, C.c_int, C.POINTER(C.c_longlong), PD, PD]
+    lib.cm_system_kernel_stats.restype = C.c_int
     for f in ("cm_system_destroy", "cm_system_set_rhs", "cm_system_set_x", "cm_system_get_x",
               "cm_system_solve", "cm_system_spmv", "cm_system_apply"):
         getattr(lib, f).restype = C.c_int
@@ -408,6 +413,27 @@ class System:
 
     def nnz(self, which):
         return self._lib.cm_system_nnz(self._h, which.encode())
+
+    KERNEL_CATS = ("A", "F1", "F2", "LU", "VEC", "SETUP", "FIN")
+
+    def profile(self, on=True, cats=None):
+        """Bracket launches with HIP events: all categories, or only `cats`."""
+        if not on:
+            mask = 0
+        elif cats is None:
+            mask = (1 << len(self.KERNEL_CATS)) - 1
+        else:
+            mask = sum(1 << self.KERNEL_CATS.index(c) for c in cats)
+        _check(self._lib.cm_system_profile(self._h, mask))
+
+    def kernel_stats(self):
+        """{category: (launches, total_ms, total_bytes)} for the profiled solves."""
+        out = {}
+        for c, name in enumerate(self.KERNEL_CATS):
+            n, ms, by = C.c_longlong(), C.c_double(), C.c_double()
+            _check(self._lib.cm_system_kernel_stats(self._h, c, C.byref(n), C.byref(ms), C.byref(by)))
+            out[name] = (n.value, ms.value, by.value)
+        return out
 
 
 def afiim_epbicgstab(A: Csr, b, F: Factors, x0=None, tol=1e-8, nmax=500, verbose=0) -> SolveResult:

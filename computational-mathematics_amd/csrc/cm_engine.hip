@@ -188,6 +188,78 @@ static inline int vec_grid(int n)
     return std::max(1, std::min(g, MAX_RED_BLOCKS));
 }
 
+// Kernel categories of the per-kernel timing (cm_system_kernel_stats).
+enum KernelCat { KC_A = 0, KC_F1 = 1, KC_F2 = 2, KC_LU = 3, KC_VEC = 4, KC_SETUP = 5, KC_FIN = 6, KC_N = 7 };
+
+// Optional per-launch timing: an event pair around every launch of a solve,
+// tagged with its category, the BiCGSTAB pass it belongs to and its
+// algorithmic bytes.  Launches the device skipped (after the exit) are
+// dropped when the solve ends, so the averages cover executed kernels only.
+struct Prof {
+    bool on = false;
+    unsigned mask = 0;    // categories whose launches are bracketed
+    int pass = -1;        // pass being enqueued (-1: setup)
+    bool p_update = false;
+    struct Rec {
+        int cat, pass;
+        bool p_update;
+        double bytes;
+        hipEvent_t e0, e1;
+    };
+    std::vector<Rec> recs;
+    std::vector<hipEvent_t> pool;
+    long long launches[KC_N] = {};
+    double ms[KC_N] = {}, bytes[KC_N] = {};
+
+    hipEvent_t get()
+    {
+        if (pool.empty()) {
+            hipEvent_t e;
+            if (hipEventCreate(&e) != hipSuccess) return nullptr;
+            return e;
+        }
+        hipEvent_t e = pool.back();
+        pool.pop_back();
+        return e;
+    }
+    void reset_stats()
+    {
+        for (int c = 0; c < KC_N; c++) {
+            launches[c] = 0;
+            ms[c] = 0.0;
+            bytes[c] = 0.0;
+        }
+    }
+    // Fold the finished solve's records into the totals.  A pass executes
+    // fully iff it computed its residual (pass < ntrace); its p update runs
+    // only when the loop went on (pass < ntrace - 1).
+    void collect(int ntrace)
+    {
+        for (Rec &r : recs) {
+            bool ran = (r.pass < 0) || (r.pass < ntrace && (!r.p_update || r.pass < ntrace - 1));
+            if (ran && r.e0 && r.e1) {
+                float t = 0.f;
+                if (hipEventElapsedTime(&t, r.e0, r.e1) == hipSuccess) {
+                    launches[r.cat]++;
+                    ms[r.cat] += t;
+                    bytes[r.cat] += r.bytes;
+                }
+            }
+            if (r.e0) pool.push_back(r.e0);
+            if (r.e1) pool.push_back(r.e1);
+        }
+        recs.clear();
+    }
+    ~Prof()
+    {
+        for (Rec &r : recs) {
+            if (r.e0) (void)hipEventDestroy(r.e0);
+            if (r.e1) (void)hipEventDestroy(r.e1);
+        }
+        for (hipEvent_t e : pool) (void)hipEventDestroy(e);
+    }
+};
+
 struct System {
     int variant = 0, n = 0, nr = 1;
     char mode = 'E';
@@ -205,6 +277,7 @@ struct System {
     double *trace = nullptr;
     int trace_cap_dev = 0;
     hipEvent_t ev[2] = {nullptr, nullptr};
+    Prof prof;
 
     ~System()
     {
@@ -237,12 +310,43 @@ struct System {
         }
     }
 
+    // Launch f on the stream, bracketed by timing events when profiling.
+    template <class F>
+    void timed(int cat, double bytes, F f)
+    {
+        if (!prof.on || !(prof.mask & (1u << cat))) {
+            f();
+            return;
+        }
+        Prof::Rec rec{cat, prof.pass, prof.p_update, bytes, prof.get(), prof.get()};
+        if (rec.e0) (void)hipEventRecord(rec.e0, stream);
+        f();
+        if (rec.e1) (void)hipEventRecord(rec.e1, stream);
+        prof.recs.push_back(rec);
+    }
+
+    // Compulsory bytes of one SpMV launch: values + column indices once, row
+    // pointers and row blocks once, x once, y written once, plus the
+    // epilogue's extra n-vectors.
     template <class Epi>
-    void spmv(const DevCsr &m, const double *xin, Epi epi, bool use_state)
+    double spmv_bytes(const DevCsr &m) const
+    {
+        return (double)m.nnz * (sizeof(double) + sizeof(int)) + 4.0 * (m.n + 1) + 4.0 * (m.nblk + 1) +
+               8.0 * m.n * (2 + Epi::XV);
+    }
+
+    template <class Epi>
+    void spmv(const DevCsr &m, const double *xin, Epi epi, bool use_state, int cat = KC_A)
     {
         if (m.nblk == 0) return;
-        spmv_kernel<double, double, Epi><<<m.nblk, BS, 0, stream>>>(
-            m.ptr, m.col, m.val, nullptr, m.blk, xin, epi, use_state ? st : nullptr, partials);
+        timed(cat, spmv_bytes<Epi>(m), [&] {
+            spmv_kernel<double, double, Epi><<<m.nblk, BS, 0, stream>>>(
+                m.ptr, m.col, m.val, nullptr, m.blk, xin, epi, use_state ? st : nullptr, partials);
+        });
+        if constexpr (Epi::K > 0)
+            timed(KC_FIN, 8.0 * Epi::K * m.nblk, [&] {
+                k_finalize<Epi::K, Epi><<<1, FBS, 0, stream>>>(m.nblk, partials, st, epi);
+            });
     }
 
     // out = M in, per the reference's apply (ACIFIM/src/bicgstab.cpp:372-428);
@@ -251,39 +355,46 @@ struct System {
     {
         const DevCsr &Hm = sym ? GT : H;
         if (variant != 2 || mode == 'E') {
-            spmv(Hm, in, EpiScale<double, double>{tm, idv}, use_state);
-            spmv(G, tm, EpiStore<double>{out}, use_state);
+            spmv(Hm, in, EpiScale<double, double>{tm, idv}, use_state, KC_F1);
+            spmv(G, tm, EpiStore<double>{out}, use_state, KC_F2);
             return;
         }
         // mode C: t1 = H b; repeat: t1 += H (b - L t1); t2 = t1 .* IDv;
         //         x = G t2; repeat: x += G (t2 - U x)
         if (nr <= 1) {
-            spmv(H, in, EpiScale<double, double>{t2, idv}, use_state);
+            spmv(H, in, EpiScale<double, double>{t2, idv}, use_state, KC_F1);
         } else {
-            spmv(H, in, EpiStore<double>{t1}, use_state);
+            spmv(H, in, EpiStore<double>{t1}, use_state, KC_F1);
             for (int it = 1; it < nr; it++) {
-                spmv(L, t1, EpiSubFrom<double>{t3, in}, use_state);
-                if (it + 1 < nr) spmv(H, t3, EpiAccum<double>{t1}, use_state);
-                else spmv(H, t3, EpiAccumScale<double, double>{t1, t2, idv}, use_state);
+                spmv(L, t1, EpiSubFrom<double>{t3, in}, use_state, KC_LU);
+                if (it + 1 < nr) spmv(H, t3, EpiAccum<double>{t1}, use_state, KC_F1);
+                else spmv(H, t3, EpiAccumScale<double, double>{t1, t2, idv}, use_state, KC_F1);
             }
         }
-        spmv(G, t2, EpiStore<double>{out}, use_state);
+        spmv(G, t2, EpiStore<double>{out}, use_state, KC_F2);
         for (int it = 1; it < nr; it++) {
-            spmv(U, out, EpiSubFrom<double>{t3, t2}, use_state);
-            spmv(G, t3, EpiAccum<double>{out}, use_state);
+            spmv(U, out, EpiSubFrom<double>{t3, t2}, use_state, KC_LU);
+            spmv(G, t3, EpiAccum<double>{out}, use_state, KC_F2);
         }
     }
 
     void enqueue_iteration()
     {
         const int g = vec_grid(n);
+        const double nn = (double)n;
+        prof.pass++;
         enqueue_apply(p, pc, true);
-        spmv(A, pc, EpiAlpha{v, r0}, true);
-        k_s_update<<<g, BS, 0, stream>>>(n, r, v, s, st);
+        spmv(A, pc, EpiAlpha{v, r0}, true, KC_A);
+        timed(KC_VEC, 24.0 * nn, [&] { k_s_update<<<g, BS, 0, stream>>>(n, r, v, s, st); });
         enqueue_apply(s, sc, true);
-        spmv(A, sc, EpiOmega{t, s}, true);
-        k_xr_update<<<g, BS, 0, stream>>>(n, x, pc, sc, r, s, t, r0, st, partials);
-        k_p_update<<<g, BS, 0, stream>>>(n, p, r, v, st);
+        spmv(A, sc, EpiOmega{t, s}, true, KC_A);
+        timed(KC_VEC, 72.0 * nn, [&] {
+            k_xr_update<<<g, BS, 0, stream>>>(n, x, pc, sc, r, s, t, r0, st, partials);
+        });
+        timed(KC_FIN, 32.0 * g, [&] { k_finalize<4, FinPass><<<1, FBS, 0, stream>>>(g, partials, st, FinPass{}); });
+        prof.p_update = true;
+        timed(KC_VEC, 32.0 * nn, [&] { k_p_update<<<g, BS, 0, stream>>>(n, p, r, v, st); });
+        prof.p_update = false;
     }
 };
 
@@ -380,34 +491,75 @@ static int system_solve(System *S, double tol, int NMAX, double *resval, int *it
     init.trace_cap = (trace && trace_cap > 0) ? trace_cap : 0;
     S->hst[0] = init;
     CM_HIP(hipMemcpyAsync(S->st, &S->hst[0], sizeof(SolveState), hipMemcpyHostToDevice, stream));
-    S->spmv(S->A, S->x, EpiSetup{S->r, S->r0, S->p, S->b}, true);
+    S->prof.pass = -1;
+    S->spmv(S->A, S->x, EpiSetup{S->r, S->r0, S->p, S->b}, true, KC_SETUP);
 
-    // chunks of iterations with one chunk in flight
-    int chunk = 2, k = 0;
-    auto enqueue_chunk = [&](int slot) -> int {
-        for (int i = 0; i < chunk; i++) S->enqueue_iteration();
+    // Passes are enqueued in chunks, at most two chunks outstanding, and the
+    // state is read back at the end of each chunk.  The next chunk's size is
+    // the number of passes the residual's recent contraction rate says are
+    // still needed (at least 4, at most 32), so few no-op passes are
+    // enqueued past the exit.
+    struct Outstanding {
+        int slot;
+        long long upto;  // passes enqueued up to the end of this chunk
+    };
+    Outstanding q[2];
+    int nq = 0, slot = 0, last_chunk = 4;
+    long long enq = 0;
+    auto push = [&](int size) -> int {
+        for (int i = 0; i < size; i++) S->enqueue_iteration();
         CM_HIP(hipMemcpyAsync(&S->hst[1 + slot], S->st, sizeof(SolveState), hipMemcpyDeviceToHost, stream));
         CM_HIP(hipEventRecord(S->ev[slot], stream));
-        chunk = std::min(chunk * 2, 32);
+        enq += size;
+        q[nq++] = Outstanding{slot, enq};
+        slot ^= 1;
+        last_chunk = size;
         return CM_OK;
     };
     int rc;
-    if ((rc = enqueue_chunk(0))) return rc;
-    int latest = 0;
-    // every pass either exits or advances iter, so NMAX passes always end it
-    const long long max_chunks = (long long)std::max(NMAX, 0) + 4;
-    for (long long c = 0; c < max_chunks; c++) {
-        const int cur = k & 1;
-        if ((rc = enqueue_chunk(cur ^ 1))) return rc;
-        latest = cur ^ 1;
-        CM_HIP(hipEventSynchronize(S->ev[cur]));
-        if (S->hst[1 + cur].done) break;
-        k++;
+    const long long nmax_passes = std::max(NMAX, 0);
+    if ((rc = push((int)std::min<long long>(4, std::max<long long>(nmax_passes, 1))))) return rc;
+    SolveState fin_state;
+    bool have_prev = false;
+    int prev_iter = 0;
+    double prev_rc = 0.0;
+    for (;;) {
+        if (nq < 2 && enq < nmax_passes) {
+            long long want = std::min(2 * last_chunk, 32);
+            if (have_prev) {
+                // estimate from the last two snapshots
+                const SolveState &sn = fin_state;
+                if (sn.iter > prev_iter && sn.rc < prev_rc && sn.rc > 0.0 && prev_rc > 0.0) {
+                    const double rate = std::pow(sn.rc / prev_rc, 1.0 / (sn.iter - prev_iter));
+                    if (rate < 1.0) {
+                        const double need = std::ceil(std::log(sn.tol * sn.ro / sn.rc) / std::log(rate)) + 1.0;
+                        const long long ahead = enq - sn.iter;
+                        want = (long long)std::max(4.0, std::min(32.0, need - (double)ahead));
+                    }
+                }
+            }
+            want = std::min(want, nmax_passes - enq);
+            if (want > 0 && (rc = push((int)want))) return rc;
+        }
+        if (nq == 0) return fail(CM_ERR_HIP, "solve did not reach an exit");
+        const Outstanding o = q[0];
+        q[0] = q[1];
+        nq--;
+        CM_HIP(hipEventSynchronize(S->ev[o.slot]));
+        const SolveState snap = S->hst[1 + o.slot];
+        if (have_prev || snap.iter > 0) {
+            prev_iter = have_prev ? fin_state.iter : 0;
+            prev_rc = have_prev ? fin_state.rc : snap.ro;
+        }
+        fin_state = snap;
+        have_prev = true;
+        if (snap.done) break;
     }
     CM_HIP(hipStreamSynchronize(stream));
     CM_HIP(hipGetLastError());
-    // the latest copy is the final state (kernels after the exit are no-ops)
-    const SolveState &fin = S->hst[1 + latest];
+    // the state is final once done is set (later kernels are no-ops)
+    const SolveState &fin = fin_state;
+    if (S->prof.on) S->prof.collect(fin.ntrace);
     if (!fin.done) return fail(CM_ERR_HIP, "solve state did not reach done");
     *iter = fin.iter;
     if (info) *info = fin.info;
@@ -543,7 +695,10 @@ int cm_system_spmv(cm_system h, char which, const double *x, double *y)
     const DevCsr *m = S.mat(which);
     if (!m || !m->present()) return fail(CM_ERR_ARG, "matrix not present");
     CM_HIP(hipMemcpyAsync(S.t1, x, (size_t)S.n * sizeof(double), hipMemcpyHostToDevice, S.stream));
+    const bool prof_on = S.prof.on;  // kernel-level calls are not part of a solve's timing
+    S.prof.on = false;
     S.spmv(*m, S.t1, EpiStore<double>{S.t2}, false);
+    S.prof.on = prof_on;
     CM_HIP(hipMemcpyAsync(y, S.t2, (size_t)S.n * sizeof(double), hipMemcpyDeviceToHost, S.stream));
     CM_HIP(hipStreamSynchronize(S.stream));
     CM_HIP(hipGetLastError());
@@ -555,7 +710,10 @@ int cm_system_apply(cm_system h, const double *v, double *out)
     if (!h || !v || !out) return fail(CM_ERR_ARG, "null argument");
     System &S = h->sys;
     CM_HIP(hipMemcpyAsync(S.s, v, (size_t)S.n * sizeof(double), hipMemcpyHostToDevice, S.stream));
+    const bool prof_on = S.prof.on;
+    S.prof.on = false;
     S.enqueue_apply(S.s, S.sc, false);
+    S.prof.on = prof_on;
     CM_HIP(hipMemcpyAsync(out, S.sc, (size_t)S.n * sizeof(double), hipMemcpyDeviceToHost, S.stream));
     CM_HIP(hipStreamSynchronize(S.stream));
     CM_HIP(hipGetLastError());
@@ -567,6 +725,26 @@ long long cm_system_nnz(cm_system h, char which)
     if (!h) return -1;
     const DevCsr *m = h->sys.mat(which);
     return (m && m->present()) ? m->nnz : -1;
+}
+
+int cm_system_profile(cm_system h, int mask)
+{
+    if (!h) return fail(CM_ERR_ARG, "null handle");
+    h->sys.prof.on = (mask != 0);
+    h->sys.prof.mask = (unsigned)mask;
+    h->sys.prof.reset_stats();
+    return CM_OK;
+}
+
+int cm_system_kernel_stats(cm_system h, int category, long long *launches, double *total_ms,
+                           double *total_bytes)
+{
+    if (!h || category < 0 || category >= KC_N) return fail(CM_ERR_ARG, "bad category");
+    const Prof &p = h->sys.prof;
+    if (launches) *launches = p.launches[category];
+    if (total_ms) *total_ms = p.ms[category];
+    if (total_bytes) *total_bytes = p.bytes[category];
+    return CM_OK;
 }
 
 int cm_afiim_epbicgstab(int n, double *Av, int *Aj, int *Ai, double *b, double tol, int NMAX,

@@ -14,9 +14,11 @@
 //  * Epilogues are fused into the SpMV row write: D^{-1} scaling (factor
 //    apply), b - A x (residuals), beta = 1 accumulation (Richardson sweeps),
 //    and the dot products BiCGSTAB needs right after A-products.
-//  * Reductions are deterministic: each workgroup writes its partial sums, the
-//    last workgroup to arrive (agent-scope acq_rel counter) adds the partials
-//    in a fixed order and updates the solve state.  No float atomics.
+//  * Reductions are deterministic: each workgroup writes its partial sums and
+//    a one-workgroup finalize kernel adds them in a fixed order and updates
+//    the solve state.  No float atomics, no cross-workgroup fences: the
+//    kernel boundary publishes the partials (an agent-scope release fence per
+//    workgroup writes back the XCD's L2 and costs more than the SpMV).
 //  * The Krylov scalars and the exit decision live in device memory
 //    (SolveState), so the host can enqueue many iterations without reading
 //    anything back; every kernel returns at once when the state says done.
@@ -40,7 +42,6 @@ struct SolveState {
     double tol, eps;
     int iter, nmax, stag, msteps, maxmsteps, info, done, variant;
     int ntrace, trace_cap;
-    unsigned int arrive[4];  // arrival counters of the last-block reductions
     double *trace;           // device array of trace_cap doubles or null
 };
 
@@ -54,9 +55,9 @@ __device__ __forceinline__ T wave_sum(T v)
     return v;
 }
 
-// Sum of K values over the workgroup; result valid on thread 0.
-template <int K>
-__device__ __forceinline__ void block_sum(double (&v)[K], double *lds /* K*BS/WAVE */)
+// Sum of K values over a workgroup of NT threads; result valid on thread 0.
+template <int K, int NT = BS>
+__device__ __forceinline__ void block_sum(double (&v)[K], double *lds /* K*NT/WAVE */)
 {
     const int lane = threadIdx.x & (WAVE - 1), w = threadIdx.x / WAVE;
 #pragma unroll
@@ -70,47 +71,68 @@ __device__ __forceinline__ void block_sum(double (&v)[K], double *lds /* K*BS/WA
 #pragma unroll
         for (int k = 0; k < K; k++) {
             double s = 0.0;
-            for (int q = 0; q < BS / WAVE; q++) s += lds[q * K + k];
+            for (int q = 0; q < NT / WAVE; q++) s += lds[q * K + k];
             v[k] = s;
         }
 }
 
-// Writes this workgroup's K partials; returns true (on every thread) in the
-// last workgroup to arrive, which then holds the grid totals on thread 0.
-template <int K>
-__device__ __forceinline__ bool grid_sum_last(double (&v)[K], double *partials,
-                                              unsigned int *arrive, double *lds)
+// Workgroup index -> row block, XCD-aware.  Consecutive workgroups are
+// dispatched round-robin over the 8 XCDs; remapping gives each XCD one
+// contiguous range of row blocks (walked in order), so the x entries that
+// neighbouring rows share stay in that XCD's L2.  A bijection of [0, nblk).
+__device__ __forceinline__ int xcd_block(int i, int nblk)
 {
-    __shared__ int am_last;
+    constexpr int NX = 8;
+    const int q = nblk / NX, rem = nblk % NX;
+    const int x = i % NX, k = i / NX;
+    return x * q + min(x, rem) + k;
+}
+
+// Writes this workgroup's K sums to partials[blockIdx.x*K + k] (plain stores;
+// the finalize kernel that reads them runs after this kernel ends, so the
+// kernel boundary makes them visible on every XCD without per-workgroup L2
+// write-back fences).
+template <int K>
+__device__ __forceinline__ void block_partials(double (&v)[K], double *partials, double *lds, int slot)
+{
     block_sum<K>(v, lds);
-    if (threadIdx.x == 0) {
+    if (threadIdx.x == 0)
 #pragma unroll
-        for (int k = 0; k < K; k++)
-            __hip_atomic_store(&partials[(size_t)blockIdx.x * K + k], v[k], __ATOMIC_RELAXED,
-                               __HIP_MEMORY_SCOPE_AGENT);
-        __threadfence();
-        unsigned int prev = __hip_atomic_fetch_add(arrive, 1u, __ATOMIC_ACQ_REL,
-                                                   __HIP_MEMORY_SCOPE_AGENT);
-        am_last = (prev == gridDim.x - 1);
-    }
-    __syncthreads();
-    if (!am_last) return false;
-    __threadfence();
+        for (int k = 0; k < K; k++) partials[(size_t)slot * K + k] = v[k];
+}
+
+// One workgroup of FBS threads: adds the nblk partials in a fixed order
+// (deterministic) and hands the totals to fin.fin(state, totals).  The loads
+// of a thread are independent, so the workgroup has all of them in flight.
+constexpr int FBS = 1024;
+
+template <int K, class Fin>
+__global__ __launch_bounds__(FBS) void k_finalize(int nblk, const double *__restrict__ partials,
+                                                  SolveState *st, Fin fin)
+{
+    if (st->done) return;
+    __shared__ double red[K * FBS / WAVE];
     double s[K];
 #pragma unroll
     for (int k = 0; k < K; k++) s[k] = 0.0;
-    for (int b = threadIdx.x; b < (int)gridDim.x; b += BS)
+    int b = threadIdx.x;
+    for (; b + 3 * FBS < nblk; b += 4 * FBS) {
+        double a0[K], a1[K], a2[K], a3[K];
 #pragma unroll
-        for (int k = 0; k < K; k++)
-            s[k] += __hip_atomic_load(&partials[(size_t)b * K + k], __ATOMIC_RELAXED,
-                                      __HIP_MEMORY_SCOPE_AGENT);
-    block_sum<K>(s, lds);
-    if (threadIdx.x == 0) {
+        for (int k = 0; k < K; k++) {
+            a0[k] = partials[(size_t)b * K + k];
+            a1[k] = partials[(size_t)(b + FBS) * K + k];
+            a2[k] = partials[(size_t)(b + 2 * FBS) * K + k];
+            a3[k] = partials[(size_t)(b + 3 * FBS) * K + k];
+        }
 #pragma unroll
-        for (int k = 0; k < K; k++) v[k] = s[k];
-        __hip_atomic_store(arrive, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        for (int k = 0; k < K; k++) s[k] += ((a0[k] + a1[k]) + (a2[k] + a3[k]));
     }
-    return true;
+    for (; b < nblk; b += FBS)
+#pragma unroll
+        for (int k = 0; k < K; k++) s[k] += partials[(size_t)b * K + k];
+    block_sum<K, FBS>(s, red);
+    if (threadIdx.x == 0) fin.fin(st, s);
 }
 
 // ------------------------------------------------------------ SpMV epilogues
@@ -122,6 +144,7 @@ __device__ __forceinline__ bool grid_sum_last(double (&v)[K], double *partials,
 template <class Y>
 struct EpiStore {  // y = A x
     static constexpr int K = 0;
+    static constexpr int XV = 0;  // n-vectors touched besides x and the y write
     Y *y;
     __device__ void row(int r, double s, double *) const { y[r] = (Y)s; }
     __device__ void fin(SolveState *, const double *) const {}
@@ -130,6 +153,7 @@ struct EpiStore {  // y = A x
 template <class Y, class D>
 struct EpiScale {  // y = d .* (A x)   (factor apply: D^{-1} after H)
     static constexpr int K = 0;
+    static constexpr int XV = 1;  // n-vectors touched besides x and the y write
     Y *y;
     const D *d;
     __device__ void row(int r, double s, double *) const { y[r] = (Y)((double)d[r] * (double)(Y)s); }
@@ -139,6 +163,7 @@ struct EpiScale {  // y = d .* (A x)   (factor apply: D^{-1} after H)
 template <class Y>
 struct EpiSubFrom {  // y = b - A x
     static constexpr int K = 0;
+    static constexpr int XV = 1;  // n-vectors touched besides x and the y write
     Y *y;
     const Y *b;
     __device__ void row(int r, double s, double *) const { y[r] = (Y)((double)b[r] - (double)(Y)s); }
@@ -148,6 +173,7 @@ struct EpiSubFrom {  // y = b - A x
 template <class Y>
 struct EpiAccum {  // y = A x + y
     static constexpr int K = 0;
+    static constexpr int XV = 1;  // n-vectors touched besides x and the y write
     Y *y;
     __device__ void row(int r, double s, double *) const { y[r] = (Y)((double)(Y)s + (double)y[r]); }
     __device__ void fin(SolveState *, const double *) const {}
@@ -156,6 +182,7 @@ struct EpiAccum {  // y = A x + y
 template <class Y, class D>
 struct EpiAccumScale {  // z = (A x + y) .* d   (last Richardson sweep then D^{-1})
     static constexpr int K = 0;
+    static constexpr int XV = 2;  // n-vectors touched besides x and the y write
     const Y *y;
     Y *z;
     const D *d;
@@ -171,6 +198,7 @@ struct EpiAccumScale {  // z = (A x + y) .* d   (last Richardson sweep then D^{-
 // (AFIIM/src/bicgstab.cpp:149-165).
 struct EpiSetup {
     static constexpr int K = 1;
+    static constexpr int XV = 3;  // n-vectors touched besides x and the y write
     double *r, *r0, *p;
     const double *b;
     __device__ void row(int i, double s, double *acc) const
@@ -201,6 +229,7 @@ struct EpiSetup {
 // ACIFIM/src/bicgstab.cpp:205-213 adds the non-finite guard).
 struct EpiAlpha {
     static constexpr int K = 1;
+    static constexpr int XV = 1;  // n-vectors touched besides x and the y write
     double *v;
     const double *r0;
     __device__ void row(int i, double s, double *acc) const
@@ -223,6 +252,7 @@ struct EpiAlpha {
 // t = A sc, omega = (t . s) / (t . t)   (AFIIM/src/bicgstab.cpp:193-198).
 struct EpiOmega {
     static constexpr int K = 2;
+    static constexpr int XV = 1;  // n-vectors touched besides x and the y write
     double *t;
     const double *s;
     __device__ void row(int i, double v, double *acc) const
@@ -240,6 +270,10 @@ template <class V>
 __device__ __forceinline__ double ldv(const V *p, int k) { return (double)p[k]; }
 template <>
 __device__ __forceinline__ double ldv<__half>(const __half *p, int k) { return (double)__half2float(p[k]); }
+template <class V>
+__device__ __forceinline__ double ldv_nt(const V *p, int k) { return (double)__builtin_nontemporal_load(&p[k]); }
+template <>
+__device__ __forceinline__ double ldv_nt<__half>(const __half *p, int k) { return (double)__half2float(p[k]); }
 
 // y = A x (+ epilogue).  A: CSR ptr/col/val; blk: row-block starts (nblk + 1).
 // Accumulation is in double for every value/vector type (the FP64 path and
@@ -261,7 +295,8 @@ __global__ __launch_bounds__(BS) void spmv_kernel(const int *__restrict__ ptr,
     for (int k = 0; k < (Epi::K > 0 ? Epi::K : 1); k++) acc[k] = 0.0;
 
     const int tid = threadIdx.x;
-    const int r0 = blk[blockIdx.x], r1 = blk[blockIdx.x + 1];
+    const int bi = xcd_block(blockIdx.x, gridDim.x);
+    const int r0 = blk[bi], r1 = blk[bi + 1];
     const int rs = ptr[r0], re = ptr[r1];
     const int cnt = re - rs;
     const int nr = r1 - r0;
@@ -283,9 +318,9 @@ __global__ __launch_bounds__(BS) void spmv_kernel(const int *__restrict__ ptr,
 #pragma unroll
         for (int u = 0; u < TILE / BS; u++) {
             const int k = tid + u * BS;
-            if (k < cnt) {
-                cidx[u] = col[rs + k];
-                cv[u] = ldv(val, rs + k);
+            if (k < cnt) {  // streamed once: nontemporal, keeps x in the caches
+                cidx[u] = __builtin_nontemporal_load(&col[rs + k]);
+                cv[u] = ldv_nt(val, rs + k);
             }
         }
 #pragma unroll
@@ -313,9 +348,7 @@ __global__ __launch_bounds__(BS) void spmv_kernel(const int *__restrict__ ptr,
             }
         }
     }
-    if constexpr (Epi::K > 0) {
-        if (grid_sum_last<Epi::K>(acc, partials, &st->arrive[0], red) && tid == 0) epi.fin(st, acc);
-    }
+    if constexpr (Epi::K > 0) block_partials<Epi::K>(acc, partials, red, bi);
 }
 
 // ----------------------------------------------------------- vector kernels
@@ -332,10 +365,8 @@ __global__ __launch_bounds__(BS) void k_s_update(int n, const double *__restrict
 
 // One pass over x, pc, sc, r, s, t, r0:
 //   ||pc||^2, ||x_old||^2, ||r_old||^2;  x += alpha pc;  x += omega sc;
-//   r = s - omega t;  r0 . r_new.
-// The last workgroup applies the exit tests in the reference's order
-// (divergence, stagnation, convergence on the lagged ||r||) and, when the
-// loop goes on, forms beta (AFIIM/src/bicgstab.cpp:200-274).
+//   r = s - omega t;  r0 . r_new   (AFIIM/src/bicgstab.cpp:200-264).
+// The partial sums go to FinPass.
 __global__ __launch_bounds__(BS) void k_xr_update(int n, double *__restrict__ x,
                                                   const double *__restrict__ pc,
                                                   const double *__restrict__ sc,
@@ -361,7 +392,15 @@ __global__ __launch_bounds__(BS) void k_xr_update(int n, double *__restrict__ x,
         r[i] = rn;
         acc[3] += r0[i] * rn;
     }
-    if (grid_sum_last<4>(acc, partials, &st->arrive[1], red) && threadIdx.x == 0) {
+    block_partials<4>(acc, partials, red, blockIdx.x);
+}
+
+// Exit tests of one pass on the grid totals of k_xr_update, in the
+// reference's order (divergence, stagnation, convergence on the lagged
+// ||r||); when the loop goes on, beta and the next pass.
+struct FinPass {
+    __device__ void fin(SolveState *st, const double *acc) const
+    {
         const double pcn = sqrt(acc[0]), xnrm = sqrt(acc[1]);
         if (fabs(st->alpha) * pcn < st->eps * xnrm) st->stag++;
         else st->stag = 0;
@@ -391,7 +430,7 @@ __global__ __launch_bounds__(BS) void k_xr_update(int n, double *__restrict__ x,
             }
         }
     }
-}
+};
 
 // p = (-beta omega) v + beta p;  p = p + r   (AFIIM/src/bicgstab.cpp:269-271)
 __global__ __launch_bounds__(BS) void k_p_update(int n, double *__restrict__ p,

@@ -92,6 +92,19 @@ int cm_system_solve(cm_system sys, double tol, int NMAX, double *resval, int *it
 int cm_system_spmv(cm_system sys, char which, const double *x, double *y);
 int cm_system_apply(cm_system sys, const double *v, double *out);
 
+/* Per-kernel timing of subsequent solves: every launch of a category whose
+ * bit is set in mask (0 = off) is bracketed by HIP events on the system's
+ * stream (each bracket costs a few microseconds of dispatch); launches the
+ * device skipped after the exit are not counted.  Statistics accumulate until
+ * the next cm_system_profile call.  Categories: 0 A products (with
+ * their fused dots), 1 first factor product (H, or G^T, with D^{-1}), 2 G
+ * products, 3 L/U products (mode C), 4 vector updates, 5 setup, 6 the
+ * one-workgroup reduction finalizers.  total_bytes
+ * is the algorithmic (compulsory) traffic of the counted launches. */
+int cm_system_profile(cm_system sys, int mask);
+int cm_system_kernel_stats(cm_system sys, int category, long long *launches, double *total_ms,
+                           double *total_bytes);
+
 /* Stored sizes (after symmetric expansion / transposition): which as above. */
 long long cm_system_nnz(cm_system sys, char which);
 

@@ -277,6 +277,7 @@ SOLVE_SYMBOLS = [
     "cm_afiim_epbicgstab", "cm_pdfiim_epbicgstab", "cm_acifim_pbicgstab", "cm_system_create",
     "cm_system_destroy", "cm_system_set_rhs", "cm_system_set_x", "cm_system_get_x", "cm_system_solve",
     "cm_system_spmv", "cm_system_apply", "cm_system_nnz", "cm_system_profile", "cm_system_kernel_stats",
+    "cm_system_enable_mixed", "cm_system_solve_mixed",
     "cm_last_error", "cm_device_count",
 ]
 
@@ -306,6 +307,10 @@ def _type_solve(lib):
     lib.cm_system_apply.argtypes = [vp, PD, PD]
     lib.cm_system_nnz.argtypes = [vp, C.c_char]
     lib.cm_system_nnz.restype = C.c_longlong
+    lib.cm_system_enable_mixed.argtypes = [vp, C.c_int]
+    lib.cm_system_enable_mixed.restype = C.c_int
+    lib.cm_system_solve_mixed.argtypes = [vp, C.c_int, C.c_double, C.c_double, C.c_int, C.c_int, PD, PI, PI, PI]
+    lib.cm_system_solve_mixed.restype = C.c_int
     lib.cm_system_profile.argtypes = [vp, C.c_int]
     lib.cm_system_profile.restype = C.c_int
     lib.cm_system_kernel_stats.argtypes = [vp, C.c_int, C.POINTER(C.c_longlong), PD, PD]
@@ -398,6 +403,18 @@ class System:
                                          _dp(tr) if trace_cap else None, trace_cap, C.byref(nt)))
         trace = tr[:min(nt.value, trace_cap)].copy() if trace_cap else None
         return resval.value, it.value, info.value, trace
+
+    def enable_mixed(self, factor_bits=32):
+        _check(self._lib.cm_system_enable_mixed(self._h, factor_bits))
+
+    def solve_mixed(self, factor_bits=32, tol=1e-8, inner_tol=1e-4, nmax=500, max_outer=30):
+        """FP64 refinement around the FP32 / FP16-factor inner solve on the
+        resident b and x.  Returns (true relative residual, outer steps,
+        inner passes, info)."""
+        rv, o, it, info = C.c_double(), C.c_int(), C.c_int(), C.c_int()
+        _check(self._lib.cm_system_solve_mixed(self._h, factor_bits, tol, inner_tol, nmax, max_outer,
+                                               C.byref(rv), C.byref(o), C.byref(it), C.byref(info)))
+        return rv.value, o.value, it.value, info.value
 
     def spmv(self, which, x):
         x = np.ascontiguousarray(x, np.float64)

@@ -6,11 +6,17 @@
 //  * CSR SpMV, CSR-Adaptive style: the host cuts the rows into blocks of
 //    consecutive rows holding at most TILE nonzeros (or one long row).  A
 //    256-thread workgroup streams its block's (value, column) pairs with fully
-//    coalesced loads, forms value*x[column] into LDS, then reduces each row
-//    with a power-of-two group of lanes (1..64) chosen from the block's row
-//    count, finishing with cross-lane xor shuffles inside the wave.  A single
-//    row longer than TILE is reduced by the whole workgroup.  Every nonzero
-//    and every column index is read exactly once.
+//    coalesced nontemporal loads, forms value*x[column] into LDS, then
+//    reduces each row with a power-of-two group of lanes (1..64) chosen from
+//    the block's row count, finishing with cross-lane xor shuffles inside the
+//    wave.  A single row longer than TILE is reduced by the whole workgroup.
+//    Every nonzero and every column index is read exactly once.  Workgroups
+//    are remapped so each XCD walks one contiguous range of rows (x reuse in
+//    its L2).
+//  * Value / vector / accumulator types are template parameters: the FP64
+//    path (double everywhere) and the mixed-precision inner solve (FP32 or
+//    FP16-with-row-scale values, FP32 vectors, FP32 row sums) share the code.
+//    Dot products and norms always accumulate in double.
 //  * Epilogues are fused into the SpMV row write: D^{-1} scaling (factor
 //    apply), b - A x (residuals), beta = 1 accumulation (Richardson sweeps),
 //    and the dot products BiCGSTAB needs right after A-products.
@@ -23,8 +29,8 @@
 //    (SolveState), so the host can enqueue many iterations without reading
 //    anything back; every kernel returns at once when the state says done.
 #pragma once
-#include <hip/hip_runtime.h>
 #include <hip/hip_fp16.h>
+#include <hip/hip_runtime.h>
 
 #include <cstdint>
 
@@ -40,9 +46,10 @@ constexpr int MAX_RED_BLOCKS = 2048;  // workgroups of the reducing vector kerne
 struct SolveState {
     double ro, rc, prc, m1, m2, alpha, omega, beta;
     double tol, eps;
+    double res_outer;  // ||b - A x|| of the last refinement residual (mixed path)
     int iter, nmax, stag, msteps, maxmsteps, info, done, variant;
     int ntrace, trace_cap;
-    double *trace;           // device array of trace_cap doubles or null
+    double *trace;  // device array of trace_cap doubles or null
 };
 
 // ---------------------------------------------------------------- reductions
@@ -88,10 +95,9 @@ __device__ __forceinline__ int xcd_block(int i, int nblk)
     return x * q + min(x, rem) + k;
 }
 
-// Writes this workgroup's K sums to partials[blockIdx.x*K + k] (plain stores;
-// the finalize kernel that reads them runs after this kernel ends, so the
-// kernel boundary makes them visible on every XCD without per-workgroup L2
-// write-back fences).
+// Writes this workgroup's K sums to partials[slot*K + k] (plain stores; the
+// finalize kernel that reads them runs after this kernel ends, so the kernel
+// boundary makes them visible on every XCD).
 template <int K>
 __device__ __forceinline__ void block_partials(double (&v)[K], double *partials, double *lds, int slot)
 {
@@ -138,77 +144,71 @@ __global__ __launch_bounds__(FBS) void k_finalize(int nblk, const double *__rest
 // ------------------------------------------------------------ SpMV epilogues
 //
 // An epilogue sees each row's sum once (row, value) and may accumulate K
-// per-row products into the workgroup's reduction; fin() runs on thread 0 of
-// the last workgroup with the grid totals.
+// per-row products into the workgroup's reduction; fin() runs in the
+// finalize kernel with the grid totals.  XV counts the n-vectors an epilogue
+// touches besides x and the y write (for the algorithmic byte count).
 
 template <class Y>
 struct EpiStore {  // y = A x
     static constexpr int K = 0;
-    static constexpr int XV = 0;  // n-vectors touched besides x and the y write
+    static constexpr int XV = 0;
     Y *y;
-    __device__ void row(int r, double s, double *) const { y[r] = (Y)s; }
+    template <class S>
+    __device__ void row(int r, S s, double *) const { y[r] = (Y)s; }
     __device__ void fin(SolveState *, const double *) const {}
 };
 
 template <class Y, class D>
 struct EpiScale {  // y = d .* (A x)   (factor apply: D^{-1} after H)
     static constexpr int K = 0;
-    static constexpr int XV = 1;  // n-vectors touched besides x and the y write
+    static constexpr int XV = 1;
     Y *y;
     const D *d;
-    __device__ void row(int r, double s, double *) const { y[r] = (Y)((double)d[r] * (double)(Y)s); }
+    template <class S>
+    __device__ void row(int r, S s, double *) const { y[r] = (Y)d[r] * (Y)s; }
     __device__ void fin(SolveState *, const double *) const {}
 };
 
 template <class Y>
 struct EpiSubFrom {  // y = b - A x
     static constexpr int K = 0;
-    static constexpr int XV = 1;  // n-vectors touched besides x and the y write
+    static constexpr int XV = 1;
     Y *y;
     const Y *b;
-    __device__ void row(int r, double s, double *) const { y[r] = (Y)((double)b[r] - (double)(Y)s); }
+    template <class S>
+    __device__ void row(int r, S s, double *) const { y[r] = b[r] - (Y)s; }
     __device__ void fin(SolveState *, const double *) const {}
 };
 
 template <class Y>
 struct EpiAccum {  // y = A x + y
     static constexpr int K = 0;
-    static constexpr int XV = 1;  // n-vectors touched besides x and the y write
+    static constexpr int XV = 1;
     Y *y;
-    __device__ void row(int r, double s, double *) const { y[r] = (Y)((double)(Y)s + (double)y[r]); }
+    template <class S>
+    __device__ void row(int r, S s, double *) const { y[r] = (Y)s + y[r]; }
     __device__ void fin(SolveState *, const double *) const {}
 };
 
 template <class Y, class D>
 struct EpiAccumScale {  // z = (A x + y) .* d   (last Richardson sweep then D^{-1})
     static constexpr int K = 0;
-    static constexpr int XV = 2;  // n-vectors touched besides x and the y write
+    static constexpr int XV = 2;
     const Y *y;
     Y *z;
     const D *d;
-    __device__ void row(int r, double s, double *) const
+    template <class S>
+    __device__ void row(int r, S s, double *) const
     {
-        Y t = (Y)((double)(Y)s + (double)y[r]);
-        z[r] = (Y)((double)t * (double)d[r]);
+        const Y t = (Y)s + y[r];
+        z[r] = t * (Y)d[r];
     }
     __device__ void fin(SolveState *, const double *) const {}
 };
 
 // Setup: r = b - A x; r0 = p = r; ro = ||r||, m1 = ro^2
 // (AFIIM/src/bicgstab.cpp:149-165).
-struct EpiSetup {
-    static constexpr int K = 1;
-    static constexpr int XV = 3;  // n-vectors touched besides x and the y write
-    double *r, *r0, *p;
-    const double *b;
-    __device__ void row(int i, double s, double *acc) const
-    {
-        const double v = b[i] + (-1.0) * s;
-        r[i] = v;
-        r0[i] = v;
-        p[i] = v;
-        acc[0] += v * v;
-    }
+struct FinSetup {
     __device__ void fin(SolveState *st, const double *tot) const
     {
         st->ro = sqrt(tot[0]);
@@ -225,17 +225,38 @@ struct EpiSetup {
     }
 };
 
+template <class T>
+struct EpiSetup {
+    static constexpr int K = 1;
+    static constexpr int XV = 3;
+    T *r, *r0, *p;
+    const T *b;
+    __device__ void fin(SolveState *st, const double *tot) const { FinSetup{}.fin(st, tot); }
+    template <class S>
+    __device__ void row(int i, S s, double *acc) const
+    {
+        const T v = b[i] + (T)(-1.0) * (T)s;
+        r[i] = v;
+        r0[i] = v;
+        p[i] = v;
+        acc[0] += (double)v * (double)v;
+    }
+};
+
 // v = A pc, alpha = m1 / (r0 . v)   (AFIIM/src/bicgstab.cpp:177-181;
 // ACIFIM/src/bicgstab.cpp:205-213 adds the non-finite guard).
+template <class T>
 struct EpiAlpha {
     static constexpr int K = 1;
-    static constexpr int XV = 1;  // n-vectors touched besides x and the y write
-    double *v;
-    const double *r0;
-    __device__ void row(int i, double s, double *acc) const
+    static constexpr int XV = 1;
+    T *v;
+    const T *r0;
+    template <class S>
+    __device__ void row(int i, S s, double *acc) const
     {
-        v[i] = s;
-        acc[0] += r0[i] * s;
+        const T vi = (T)s;
+        v[i] = vi;
+        acc[0] += (double)r0[i] * (double)vi;
     }
     __device__ void fin(SolveState *st, const double *tot) const
     {
@@ -250,35 +271,70 @@ struct EpiAlpha {
 };
 
 // t = A sc, omega = (t . s) / (t . t)   (AFIIM/src/bicgstab.cpp:193-198).
+template <class T>
 struct EpiOmega {
     static constexpr int K = 2;
-    static constexpr int XV = 1;  // n-vectors touched besides x and the y write
-    double *t;
-    const double *s;
-    __device__ void row(int i, double v, double *acc) const
+    static constexpr int XV = 1;
+    T *t;
+    const T *s;
+    template <class S>
+    __device__ void row(int i, S v, double *acc) const
     {
-        t[i] = v;
-        acc[0] += v * s[i];
-        acc[1] += v * v;
+        const T ti = (T)v;
+        t[i] = ti;
+        acc[0] += (double)ti * (double)s[i];
+        acc[1] += (double)ti * (double)ti;
     }
     __device__ void fin(SolveState *st, const double *tot) const { st->omega = tot[0] / tot[1]; }
+};
+
+// Refinement residual (FP64): r = b - A x in double, its FP32 copy for the
+// inner solve, and ||r||^2.
+struct EpiResid {
+    static constexpr int K = 1;
+    static constexpr int XV = 2;  // b read, FP32 copy written (half width)
+    const double *b;
+    float *r32;
+    template <class S>
+    __device__ void row(int i, S s, double *acc) const
+    {
+        const double v = b[i] - (double)s;
+        r32[i] = (float)v;
+        acc[0] += v * v;
+    }
+    __device__ void fin(SolveState *st, const double *tot) const { st->res_outer = sqrt(tot[0]); }
 };
 
 // ------------------------------------------------------------------- SpMV
 
 template <class V>
-__device__ __forceinline__ double ldv(const V *p, int k) { return (double)p[k]; }
+__device__ __forceinline__ float ldv_nt(const V *p, int k) { return (float)__builtin_nontemporal_load(&p[k]); }
 template <>
-__device__ __forceinline__ double ldv<__half>(const __half *p, int k) { return (double)__half2float(p[k]); }
+__device__ __forceinline__ float ldv_nt<__half>(const __half *p, int k)
+{
+    const unsigned short u = __builtin_nontemporal_load(reinterpret_cast<const unsigned short *>(p) + k);
+    return __half2float(__ushort_as_half(u));
+}
 template <class V>
-__device__ __forceinline__ double ldv_nt(const V *p, int k) { return (double)__builtin_nontemporal_load(&p[k]); }
+__device__ __forceinline__ float ldv(const V *p, int k) { return (float)p[k]; }
 template <>
-__device__ __forceinline__ double ldv_nt<__half>(const __half *p, int k) { return (double)__half2float(p[k]); }
+__device__ __forceinline__ float ldv<__half>(const __half *p, int k) { return __half2float(p[k]); }
 
-// y = A x (+ epilogue).  A: CSR ptr/col/val; blk: row-block starts (nblk + 1).
-// Accumulation is in double for every value/vector type (the FP64 path and
-// the FP32 inner solve share the kernel; FP32 inputs are exact in double).
-template <class V, class X, class Epi>
+template <class V, class Acc>
+__device__ __forceinline__ Acc load_val(const V *p, int k, bool nt)
+{
+    return nt ? (Acc)ldv_nt(p, k) : (Acc)ldv(p, k);
+}
+template <>
+__device__ __forceinline__ double load_val<double, double>(const double *p, int k, bool nt)
+{
+    return nt ? __builtin_nontemporal_load(&p[k]) : p[k];
+}
+
+// y = A x (+ epilogue).  A: CSR ptr/col/val (rowscale: optional per-row
+// factor applied to the row sum, for FP16 values stored scaled); blk:
+// row-block starts (gridDim.x + 1).  Products and row sums in Acc.
+template <class V, class X, class Acc, class Epi>
 __global__ __launch_bounds__(BS) void spmv_kernel(const int *__restrict__ ptr,
                                                   const int *__restrict__ col,
                                                   const V *__restrict__ val,
@@ -288,7 +344,7 @@ __global__ __launch_bounds__(BS) void spmv_kernel(const int *__restrict__ ptr,
                                                   SolveState *st, double *partials)
 {
     if (st && st->done) return;
-    __shared__ double prod[TILE];
+    __shared__ Acc prod[TILE];
     __shared__ double red[(Epi::K > 0 ? Epi::K : 1) * BS / WAVE];
     double acc[Epi::K > 0 ? Epi::K : 1];
 #pragma unroll
@@ -304,29 +360,29 @@ __global__ __launch_bounds__(BS) void spmv_kernel(const int *__restrict__ ptr,
     if (cnt > TILE) {
         // one long row: the whole workgroup strides over it
         double s = 0.0;
-        for (int k = rs + tid; k < re; k += BS) s += ldv(val, k) * (double)x[col[k]];
+        for (int k = rs + tid; k < re; k += BS) s += (double)(load_val<V, Acc>(val, k, false) * (Acc)x[col[k]]);
         double v1[1] = {s};
-        block_sum<1>(v1, prod);
+        block_sum<1>(v1, red);
         if (tid == 0) {
-            double rsum = v1[0];
-            if (rowscale) rsum *= (double)rowscale[r0];
+            Acc rsum = (Acc)v1[0];
+            if (rowscale) rsum *= (Acc)rowscale[r0];
             epi.row(r0, rsum, acc);
         }
     } else {
         int cidx[TILE / BS];
-        double cv[TILE / BS];
+        Acc cv[TILE / BS];
 #pragma unroll
         for (int u = 0; u < TILE / BS; u++) {
             const int k = tid + u * BS;
             if (k < cnt) {  // streamed once: nontemporal, keeps x in the caches
                 cidx[u] = __builtin_nontemporal_load(&col[rs + k]);
-                cv[u] = ldv_nt(val, rs + k);
+                cv[u] = load_val<V, Acc>(val, rs + k, true);
             }
         }
 #pragma unroll
         for (int u = 0; u < TILE / BS; u++) {
             const int k = tid + u * BS;
-            if (k < cnt) prod[k] = cv[u] * (double)x[cidx[u]];
+            if (k < cnt) prod[k] = cv[u] * (Acc)x[cidx[u]];
         }
         __syncthreads();
         int tpr = 1;  // lanes per row: a power of two, at most one wave
@@ -336,14 +392,14 @@ __global__ __launch_bounds__(BS) void spmv_kernel(const int *__restrict__ ptr,
         for (int base = 0; base < nr; base += groups) {
             const int rr = base + g;
             const bool live = rr < nr;
-            double s = 0.0;
+            Acc s = 0;
             if (live) {
                 const int a = ptr[r0 + rr] - rs, e = ptr[r0 + rr + 1] - rs;
                 for (int k = a + lane; k < e; k += tpr) s += prod[k];
             }
             for (int off = tpr / 2; off > 0; off >>= 1) s += __shfl_xor(s, off, WAVE);
             if (live && lane == 0) {
-                if (rowscale) s *= (double)rowscale[r0 + rr];
+                if (rowscale) s *= (Acc)rowscale[r0 + rr];
                 epi.row(r0 + rr, s, acc);
             }
         }
@@ -354,12 +410,12 @@ __global__ __launch_bounds__(BS) void spmv_kernel(const int *__restrict__ ptr,
 // ----------------------------------------------------------- vector kernels
 
 // s = r - alpha v   (AFIIM/src/bicgstab.cpp:183-185)
-__global__ __launch_bounds__(BS) void k_s_update(int n, const double *__restrict__ r,
-                                                 const double *__restrict__ v,
-                                                 double *__restrict__ s, const SolveState *st)
+template <class T>
+__global__ __launch_bounds__(BS) void k_s_update(int n, const T *__restrict__ r, const T *__restrict__ v,
+                                                 T *__restrict__ s, const SolveState *st)
 {
     if (st->done) return;
-    const double na = -st->alpha;
+    const T na = (T)(-st->alpha);
     for (int i = blockIdx.x * BS + threadIdx.x; i < n; i += gridDim.x * BS) s[i] = r[i] + na * v[i];
 }
 
@@ -367,30 +423,28 @@ __global__ __launch_bounds__(BS) void k_s_update(int n, const double *__restrict
 //   ||pc||^2, ||x_old||^2, ||r_old||^2;  x += alpha pc;  x += omega sc;
 //   r = s - omega t;  r0 . r_new   (AFIIM/src/bicgstab.cpp:200-264).
 // The partial sums go to FinPass.
-__global__ __launch_bounds__(BS) void k_xr_update(int n, double *__restrict__ x,
-                                                  const double *__restrict__ pc,
-                                                  const double *__restrict__ sc,
-                                                  double *__restrict__ r,
-                                                  const double *__restrict__ s,
-                                                  const double *__restrict__ t,
-                                                  const double *__restrict__ r0,
-                                                  SolveState *st, double *partials)
+template <class T>
+__global__ __launch_bounds__(BS) void k_xr_update(int n, T *__restrict__ x, const T *__restrict__ pc,
+                                                  const T *__restrict__ sc, T *__restrict__ r,
+                                                  const T *__restrict__ s, const T *__restrict__ t,
+                                                  const T *__restrict__ r0, SolveState *st,
+                                                  double *partials)
 {
     if (st->done) return;
     __shared__ double red[4 * BS / WAVE];
-    const double a = st->alpha, om = st->omega, nom = -om;
+    const T a = (T)st->alpha, om = (T)st->omega, nom = (T)(-st->omega);
     double acc[4] = {0.0, 0.0, 0.0, 0.0};
     for (int i = blockIdx.x * BS + threadIdx.x; i < n; i += gridDim.x * BS) {
-        const double pci = pc[i], xi = x[i], ri = r[i];
-        acc[0] += pci * pci;
-        acc[1] += xi * xi;
-        acc[2] += ri * ri;
-        double xn = xi + a * pci;
+        const T pci = pc[i], xi = x[i], ri = r[i];
+        acc[0] += (double)pci * (double)pci;
+        acc[1] += (double)xi * (double)xi;
+        acc[2] += (double)ri * (double)ri;
+        T xn = xi + a * pci;
         xn = xn + om * sc[i];
         x[i] = xn;
-        const double rn = s[i] + nom * t[i];
+        const T rn = s[i] + nom * t[i];
         r[i] = rn;
-        acc[3] += r0[i] * rn;
+        acc[3] += (double)r0[i] * (double)rn;
     }
     block_partials<4>(acc, partials, red, blockIdx.x);
 }
@@ -433,27 +487,43 @@ struct FinPass {
 };
 
 // p = (-beta omega) v + beta p;  p = p + r   (AFIIM/src/bicgstab.cpp:269-271)
-__global__ __launch_bounds__(BS) void k_p_update(int n, double *__restrict__ p,
-                                                 const double *__restrict__ r,
-                                                 const double *__restrict__ v, const SolveState *st)
+template <class T>
+__global__ __launch_bounds__(BS) void k_p_update(int n, T *__restrict__ p, const T *__restrict__ r,
+                                                 const T *__restrict__ v, const SolveState *st)
 {
     // runs after k_xr_update of the same pass; when that pass exited, p is dead
     if (st->done) return;
-    const double be = st->beta, c = -st->beta * st->omega;
+    const T be = (T)st->beta, c = (T)(-st->beta * st->omega);
     for (int i = blockIdx.x * BS + threadIdx.x; i < n; i += gridDim.x * BS) {
-        const double q = c * v[i] + be * p[i];
+        const T q = c * v[i] + be * p[i];
         p[i] = q + r[i];
     }
 }
 
-// z = a .* b
-template <class T, class D>
-__global__ __launch_bounds__(BS) void k_vmul(int n, const T *__restrict__ a, const D *__restrict__ d,
-                                             T *__restrict__ z, const SolveState *st)
+// Inner-solve setup from x = 0: r = r0 = p = b, ||b||^2 partials (FinSetup).
+template <class T>
+__global__ __launch_bounds__(BS) void k_setup_zero(int n, const T *__restrict__ b, T *__restrict__ x,
+                                                   T *__restrict__ r, T *__restrict__ r0, T *__restrict__ p,
+                                                   const SolveState *st, double *partials)
 {
-    if (st && st->done) return;
-    for (int i = blockIdx.x * BS + threadIdx.x; i < n; i += gridDim.x * BS)
-        z[i] = (T)((double)a[i] * (double)d[i]);
+    __shared__ double red[BS / WAVE];
+    double acc[1] = {0.0};
+    for (int i = blockIdx.x * BS + threadIdx.x; i < n; i += gridDim.x * BS) {
+        const T v = b[i];
+        x[i] = (T)0;
+        r[i] = v;
+        r0[i] = v;
+        p[i] = v;
+        acc[0] += (double)v * (double)v;
+    }
+    block_partials<1>(acc, partials, red, blockIdx.x);
+    (void)st;
+}
+
+// x64 += x32 (refinement update of the FP64 solution).
+__global__ __launch_bounds__(BS) void k_x_accum(int n, double *__restrict__ x, const float *__restrict__ d)
+{
+    for (int i = blockIdx.x * BS + threadIdx.x; i < n; i += gridDim.x * BS) x[i] += (double)d[i];
 }
 
 }  // namespace cm

@@ -86,6 +86,20 @@ int cm_system_get_x(cm_system sys, double *x);
 int cm_system_solve(cm_system sys, double tol, int NMAX, double *resval, int *iter,
                     int *info, double *trace, int trace_cap, int *ntrace);
 
+/* Mixed precision on the resident b and x: FP64 iterative refinement (true
+ * residual b - A x, the solution and its norm in FP64, with the FP64 A)
+ * around an inner BiCGSTAB on an FP32 A, FP32 (factor_bits 32) or FP16
+ * per-row-scaled (factor_bits 16) factors and FP32 vectors, solved to
+ * inner_tol from zero at every refinement step.  enable_mixed casts the
+ * low-precision copies on the device once (they share the FP64 index
+ * arrays).  Stops when ||b - A x|| <= tol ||b - A x0||: *resval is that true
+ * relative residual, *outer the refinement steps, *inner_total the inner
+ * BiCGSTAB passes; info 0 converged, 1 non-finite residual, 2 a refinement
+ * step did not reduce the residual, 3 max_outer steps reached. */
+int cm_system_enable_mixed(cm_system sys, int factor_bits);
+int cm_system_solve_mixed(cm_system sys, int factor_bits, double tol, double inner_tol, int NMAX,
+                          int max_outer, double *resval, int *outer, int *inner_total, int *info);
+
 /* Kernel-level entry points (host vectors in/out), for parity checks:
  * which = 'A', 'G', 'H', 'L', 'U' (and 'T' for the G^T copy of a symmetric
  * PDFIIM system): y = M x.  apply: out = M v with the system's preconditioner. */

@@ -160,3 +160,66 @@ def test_dropin_entry_points():
     assert r.info == info_o == 0
     assert abs(r.iter - ito) <= 1
     assert _rel(r.x, xo) <= 1e-6
+
+
+@pytest.mark.parametrize("mode", ["I", "J"])
+def test_acifim_triangular_modes(mode):
+    A = cmamd.gen_convdiff(3, 12, 5.0, seed=8)
+    xs = cmamd.gen_uniform(7, A.n)
+    b = cmamd.csr_mv_host(A, xs)
+    F = cmamd.acifim(A, 0.01, 0.1)
+    sys_ = cmamd.System("acifim", A, F, nr=1, mode=mode)
+    v = cmamd.gen_uniform(8, A.n)
+    assert _rel(sys_.apply(v), oracle.apply(F, v, mode=mode)) <= SPMV_TOL
+    sys_.close()
+    _compare_solve("acifim", A, F, b, 2000, nr=1, mode=mode)
+
+
+def test_pdfiim_symmetric_branch():
+    import scipy.sparse as sp
+    A = cmamd.gen_convdiff(3, 14, 0.0, seed=13)  # Pe = 0: symmetric
+    Up = cmamd.Csr.from_scipy(sp.triu(A.to_scipy()).tocsr())
+    F = cmamd.afiim(A, 0.1)
+    Fs = cmamd.Factors(F.G, F.idv, None)
+    b = cmamd.csr_mv_host(A, cmamd.gen_uniform(14, A.n))
+    sys_ = cmamd.System("pdfiim", Up, Fs, a_symmetric=True)
+    x = cmamd.gen_uniform(15, A.n)
+    assert _rel(sys_.spmv("A", x), oracle.csr_mv(Up, x, "sym_upper")) <= SPMV_TOL
+    sys_.set_rhs(b)
+    sys_.set_x(None)
+    rv, it, info, tr = sys_.solve(tol=1e-8, nmax=1200, trace_cap=64)
+    sys_.close()
+    xo, rvo, ito, info_o, tro = oracle.bicgstab("pdfiim", Up, b, Fs, nmax=1200, issym=True, trace_cap=64)
+    m = min(20, len(tr), len(tro))
+    assert np.max(np.abs(tr[:m] - tro[:m]) / tro[:m]) <= TRACE_TOL
+    assert info == info_o == 0 and abs(it - ito) <= max(1, int(np.ceil(0.02 * ito)))
+
+
+@pytest.mark.parametrize("bits", [32, 16])
+@pytest.mark.parametrize("case", ["c1", "c3"])
+def test_mixed_precision_refinement(bits, case):
+    if case == "c1":
+        A = cmamd.gen_convdiff(2, 100, 0.5, seed=1)
+        F = cmamd.afiim(A, 0.1)
+        variant, tol = "afiim", 1e-8
+    else:
+        A = cmamd.gen_convdiff(3, 24, 5.0, seed=4)
+        F = cmamd.acifim(A, 0.01, 0.1)
+        variant, tol = "acifim", 1e-10
+    xs = cmamd.gen_uniform(5, A.n)
+    b = cmamd.csr_mv_host(A, xs)
+    sys_ = cmamd.System(variant, A, F)
+    sys_.enable_mixed(bits)
+    sys_.set_rhs(b)
+    sys_.set_x(None)
+    rv, outer, inner, info = sys_.solve_mixed(bits, tol=tol, inner_tol=1e-4, nmax=2000)
+    x = sys_.get_x()
+    sys_.close()
+    true_res = np.linalg.norm(b - cmamd.csr_mv_host(A, x)) / np.linalg.norm(b)
+    assert info == 0, (rv, outer, inner, info)
+    assert true_res <= tol and abs(true_res - rv) <= 1e-3 * tol + 1e-2 * rv
+    xo = oracle.bicgstab(variant, A, b, F, tol=tol, nmax=2000)[0]
+    # error against the FP64 oracle solution: both are within tol of the
+    # exact solution in the residual sense
+    assert _rel(x, xo) <= 1e3 * tol
+    assert 1 <= outer <= 6

@@ -2,7 +2,8 @@
 reference solve loops), through the C ABI.
 
 Bars: SpMV and M-apply within 1e-12 relative (normwise) of the oracle; the
-per-pass residual norms within 1e-8 relative over the first 20 passes;
+per-pass residual norms within 1e-8 relative over the first 20 passes (down
+to a rounding floor of 1e-12 ||r_0||, see _trace_close);
 iteration counts within 2%; identical info codes; the true relative residual
 of the returned x meets the tolerance.
 """
@@ -20,6 +21,14 @@ TRACE_TOL = 1e-8
 
 def _rel(a, b):
     return np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300)
+
+
+def _trace_close(tr, tro, rel=TRACE_TOL, floor=1e-12):
+    """Per-pass residual norms agree within `rel` relative, down to a rounding
+    floor of `floor` * ||r_0||: two FP64 implementations that sum in different
+    orders differ by ~eps * ||r_0|| in ||r_k||, which is 1e-8 relative once the
+    residual has dropped eight orders (tro[0] is ||r_0||)."""
+    return bool(np.all(np.abs(tr - tro) <= rel * tro + floor * tro[0]))
 
 
 def _system(kind):
@@ -97,7 +106,7 @@ def _compare_solve(variant, A, F, b, nmax, tol=1e-8, **kw):
     xo, rvo, ito, info_o, tro = oracle.bicgstab(variant, A, b, F, tol=tol, nmax=nmax, trace_cap=64, **okw)
     m = min(20, len(tr), len(tro))
     assert m > 0
-    assert np.max(np.abs(tr[:m] - tro[:m]) / tro[:m]) <= TRACE_TOL
+    assert _trace_close(tr[:m], tro[:m])
     assert info == info_o
     assert abs(it - ito) <= max(1, int(np.ceil(0.02 * ito)))
     return x, rv, it, info, xo
@@ -191,7 +200,7 @@ def test_pdfiim_symmetric_branch():
     sys_.close()
     xo, rvo, ito, info_o, tro = oracle.bicgstab("pdfiim", Up, b, Fs, nmax=1200, issym=True, trace_cap=64)
     m = min(20, len(tr), len(tro))
-    assert np.max(np.abs(tr[:m] - tro[:m]) / tro[:m]) <= TRACE_TOL
+    assert _trace_close(tr[:m], tro[:m])
     assert info == info_o == 0 and abs(it - ito) <= max(1, int(np.ceil(0.02 * ito)))
 
 

@@ -1,0 +1,119 @@
+"""GPU parity on the reference's own fixture, HB/sherman2 (n=1080,
+nnz=23094; {AFIIM,PDFIIM,ACIFIM}/example/sherman2.mtx + sherman2_rhs1.mtx),
+committed as tests/golden/sherman2.npz, with the reference-built factors for
+the README parameters (tests/golden/afiim_sherman2.npz, acifim_sherman2.npz).
+
+Every solve variant and apply mode runs through the C ABI and is compared with
+the oracle on the same factors; the drop-in example driver (reference call
+signature via include/cm_dropin.hpp) is run end to end.
+
+sherman2 has a condition number near 1e12, and its BiCGSTAB iterates amplify
+rounding: the oracle against itself with 1, 2 and 8 threads (only the
+summation order of its dot products differs) already gives 17, 15 and 17
+iterations, per-pass residual norms that differ by 3e-7 at pass 7 and by 63%
+at pass 17, and solutions that differ by 7e-6.  No FP64 implementation with
+a different summation order can meet a 1e-8 trace bar over 20 passes here,
+so these tests hold the GPU to the oracle's own spread: the trace to 1e-8 for
+the first 6 passes, the same exit code, the iteration count within 3 of the
+oracle's, and the solution within 1e-4.
+"""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+import cmamd
+import oracle
+from test_gpu_parity import _trace_close
+
+pytestmark = pytest.mark.gpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+GOLDEN = os.path.join(ROOT, "tests", "golden")
+
+
+def _sherman2():
+    d = np.load(os.path.join(GOLDEN, "sherman2.npz"))
+    return cmamd.Csr(len(d["ptr"]) - 1, d["ptr"], d["idx"], d["val"]), d["rhs"]
+
+
+def _golden_afiim():
+    d = np.load(os.path.join(GOLDEN, "afiim_sherman2.npz"))
+    n = len(d["gp"]) - 1
+    return cmamd.Factors(cmamd.Csr(n, d["gp"], d["gi"], d["gv"]), d["idv"], cmamd.Csr(n, d["hp"], d["hi"], d["hv"]))
+
+
+def _golden_acifim():
+    d = np.load(os.path.join(GOLDEN, "acifim_sherman2.npz"))
+    n = len(d["gp"]) - 1
+    c = lambda p: cmamd.Csr(n, d[p + "p"], d[p + "i"], d[p + "v"])  # noqa: E731
+    return cmamd.Factors(c("g"), d["idv"], c("h"), c("l"), c("u"))
+
+
+def _check(variant, A, b, F, nmax, **kw):
+    sys_ = cmamd.System(variant, A, F, **kw)
+    sys_.set_rhs(b)
+    sys_.set_x(None)
+    rv, it, info, tr = sys_.solve(tol=1e-8, nmax=nmax, trace_cap=64)
+    x = sys_.get_x()
+    sys_.close()
+    okw = {k: v for k, v in kw.items() if k in ("nr", "mode")}
+    xo, rvo, ito, info_o, tro = oracle.bicgstab(variant, A, b, F, tol=1e-8, nmax=nmax, trace_cap=64, **okw)
+    m = min(6, len(tr), len(tro))
+    assert _trace_close(tr[:m], tro[:m])
+    assert info == info_o
+    assert abs(it - ito) <= 3
+    if info == 0:
+        assert np.linalg.norm(x - xo) / np.linalg.norm(xo) < 1e-4
+    return x, it, info
+
+
+def test_afiim_readme_case():
+    A, b = _sherman2()
+    F = _golden_afiim()  # AFIIM/README:29: eps = 9e-6
+    x, it, info = _check("afiim", A, b, F, 500)
+    assert info == 0
+    assert np.linalg.norm(b - cmamd.csr_mv_host(A, x)) / np.linalg.norm(b) < 1e-7
+
+
+@pytest.mark.parametrize("mode,nr", [("E", 1), ("C", 2), ("I", 1), ("J", 1)])
+def test_acifim_readme_case(mode, nr):
+    A, b = _sherman2()
+    F = _golden_acifim()  # ACIFIM/README:31: eps1 = eps2 = 0.01, nr = 2, mode C
+    nmax = 2000 if mode != "J" else 60  # J does not converge here (oracle: NMAX reached)
+    x, it, info = _check("acifim", A, b, F, nmax, nr=nr, mode=mode)
+    if mode != "J":
+        assert info == 0
+
+
+def test_pdfiim_on_sherman2():
+    # eps = 0.001 converges in 11 passes for every oracle thread count; at
+    # eps = 0.01 the oracle alone spreads over 170-207 passes (1-8 threads)
+    A, b = _sherman2()
+    F = cmamd.pdfiim(A, 2, 2, 1, 0.001, -1, "I", "B")
+    x, it, info = _check("pdfiim", A, b, F, 1200)
+    assert info == 0
+
+
+def test_dropin_example_binary(tmp_path):
+    exe = os.path.join(ROOT, "computational-mathematics_amd", "bin", "afiim_solve")
+    assert os.path.exists(exe), "example driver not built"
+    A, b = _sherman2()
+    pa, pb, px = tmp_path / "A.mtx", tmp_path / "b.mtx", tmp_path / "x.mtx"
+    with open(pa, "w") as f:
+        f.write("%%MatrixMarket matrix coordinate real general\n")
+        f.write(f"{A.n} {A.n} {A.nnz}\n")
+        for i in range(A.n):
+            for k in range(A.ptr[i], A.ptr[i + 1]):
+                f.write("%d %d %.17g\n" % (i + 1, A.idx[k] + 1, A.val[k]))
+    with open(pb, "w") as f:
+        f.write(f"%%MatrixMarket matrix array real general\n{A.n} 1\n")
+        for v in b:
+            f.write("%.17g\n" % v)
+    r = subprocess.run([exe, str(pa), str(pb), "9e-6", str(px)], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "Convergence in" in r.stdout
+    x = cmamd.read_mtx_vector(str(px))
+    xo = oracle.bicgstab("afiim", A, b, _golden_afiim(), nmax=500)[0]
+    assert np.linalg.norm(x - xo) / np.linalg.norm(xo) < 1e-4

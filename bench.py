@@ -37,6 +37,10 @@ WORKLOADS = {
              "velocity (1,1,1), cell Peclet 1; PDFIIM(prefill=2, lfill=2, postfill=1, eps=0.05); "
              "b=A x*, x*~U[-1,1], x0=0; plain FP64 BiCGSTAB to rel-res 1e-8",
         dim=3, N=128, pe=1.0, seed=2, xseed=3, prec="pdfiim", args=(2, 2, 1, 0.05), tol=1e-8, nmax=1200),
+    "c3": dict(
+        desc="3D 7-point convection-diffusion 128^3 (n=2097152, nnz=14581760), cell Peclet 5; "
+             "ACIFIM(eps1=0.01, eps2=0.1), apply mode E; b=A x*, x0=0; plain FP64 BiCGSTAB to rel-res 1e-10",
+        dim=3, N=128, pe=5.0, seed=4, xseed=5, prec="acifim", args=(0.01, 0.1), tol=1e-10, nmax=2000),
     "c1": dict(
         desc="2D 5-point convection-diffusion 100^2 (n=10000, nnz=49600), cell Peclet 0.5; AFIIM eps=0.1; "
              "b=A x*, x0=0; plain FP64 BiCGSTAB to rel-res 1e-8",
@@ -90,6 +94,35 @@ def cpu_baseline(w, A, b, F, seconds):
                       f"{med * 1e3:.1f} ms/solve"}
 
 
+def mixed_leg(sys_, bits, w, A, b, x64, args):
+    """Mixed precision on the same resident system: FP64 refinement around an
+    inner BiCGSTAB on FP32 A, FP32 or FP16 (per-row scaled) factors and FP32
+    vectors, to the same true relative residual as the FP64 path."""
+    import cmamd
+    sys_.enable_mixed(bits)
+    for _ in range(max(1, args.warmup)):
+        sys_.set_x(None)
+        sys_.solve_mixed(bits, tol=w["tol"], inner_tol=1e-4, nmax=w["nmax"])
+    lat = []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        ts = time.perf_counter()
+        sys_.set_x(None)
+        rv, outer, inner, info = sys_.solve_mixed(bits, tol=w["tol"], inner_tol=1e-4, nmax=w["nmax"])
+        lat.append(time.perf_counter() - ts)
+    elapsed = time.perf_counter() - t0
+    x = sys_.get_x()
+    true_res = float(np.linalg.norm(b - cmamd.csr_mv_host(A, x)) / np.linalg.norm(b))
+    return {"value": round(args.steps / elapsed, 3), "unit": "solves/s",
+            "ms_per_step": round(1e3 * elapsed / args.steps, 3),
+            "latency_ms": {"p50": round(1e3 * float(np.percentile(lat, 50)), 3),
+                           "p99": round(1e3 * float(np.percentile(lat, 99)), 3)},
+            "outer_steps": outer, "inner_passes": inner, "info": info, "true_rel_residual": true_res,
+            "rel_error_vs_fp64_solution": float(np.linalg.norm(x - x64) / np.linalg.norm(x64)),
+            "inner": f"BiCGSTAB on FP32 A, FP{bits} factors{' (per-row scaled)' if bits == 16 else ''}, "
+                     f"FP32 vectors, to 1e-4; FP64 refinement of b - A x"}
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -98,10 +131,24 @@ def main():
     ap.add_argument("--workload", default="c2", choices=sorted(WORKLOADS))
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision legs")
     ap.add_argument("--kernel-timing", choices=("timed-region", "separate"), default="timed-region",
                     help="bracket the dominant kernel's launches with HIP events inside the timed region "
                          "(default), or time the region without events and time that kernel in extra solves")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="run only a plain device copy of 1 GiB (x10), the reference kernel for scaling the "
+                         "FETCH_SIZE / WRITE_SIZE hardware counters in rocprofv3 --pmc runs")
     args = ap.parse_args()
+    if args.calibrate_copy:
+        import torch
+        src = torch.ones(2 ** 27, dtype=torch.float64, device="cuda")  # 1 GiB
+        dst = torch.empty_like(src)
+        for _ in range(10):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_read": src.numel() * 8, "copy_bytes_written": dst.numel() * 8,
+                          "launches": 10}))
+        return
     if args.gpus != 1:
         print(f"bench.py: this engine runs on one GPU; --gpus {args.gpus} is not supported "
               f"(multi-GPU sharding is a design note only)", file=sys.stderr)
@@ -150,6 +197,10 @@ def main():
     x = sys_.get_x()
     true_res = float(np.linalg.norm(b - cmamd.csr_mv_host(A, x)) / np.linalg.norm(b))
     fwd_err = float(np.linalg.norm(x - xs) / np.linalg.norm(xs))
+    mixed = {}
+    if not args.no_mixed:
+        for bits in (32, 16):
+            mixed[f"fp{bits}_factors"] = mixed_leg(sys_, bits, w, A, b, x, args)
     sys_.close()
 
     value = args.steps / elapsed
@@ -187,6 +238,7 @@ def main():
                        "p99": round(1e3 * float(np.percentile(lat, 99)), 3)},
         "iterations": int(np.median(iters)), "info": info, "resval": rv, "true_rel_residual": true_res,
         "forward_error_vs_xstar": fwd_err,
+        "mixed_precision": mixed,
         "roofline": roofline, "kernels": kernels,
         "kernels_note": "per-kernel table from 2 extra solves with every launch bracketed by HIP events; "
                         "roofline from the timed region, where only the dominant kernel is bracketed",

@@ -645,7 +645,7 @@ struct Krylov {
     // chunks, at most two chunks outstanding, and the state is read back at
     // the end of each chunk.  The next chunk's size is the number of passes
     // the residual's recent contraction rate says are still needed (at least
-    // 4, at most 32), so few no-op passes are enqueued past the exit.
+    // 2, at most 32), so few no-op passes are enqueued past the exit.
     int run(int NMAX, SolveState &fin_state)
     {
         int qs[2];
@@ -676,7 +676,7 @@ struct Krylov {
                         if (rate < 1.0) {
                             const double need = std::ceil(std::log(sn.tol * sn.ro / sn.rc) / std::log(rate)) + 1.0;
                             const long long ahead = enq - sn.iter;
-                            want = (long long)std::max(4.0, std::min(32.0, need - (double)ahead));
+                            want = (long long)std::max(2.0, std::min(32.0, need - (double)ahead));
                         }
                     }
                 }

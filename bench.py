@@ -41,6 +41,19 @@ WORKLOADS = {
         desc="3D 7-point convection-diffusion 128^3 (n=2097152, nnz=14581760), cell Peclet 5; "
              "ACIFIM(eps1=0.01, eps2=0.1), apply mode E; b=A x*, x0=0; plain FP64 BiCGSTAB to rel-res 1e-10",
         dim=3, N=128, pe=5.0, seed=4, xseed=5, prec="acifim", args=(0.01, 0.1), tol=1e-10, nmax=2000),
+    "c4": dict(
+        desc="3D 27-point stencil 100^3 (n=1000000), off-diagonals -(1+0.3 U[-1,1]), diagonal 1.05 x row "
+             "absolute sum, 5% of rows with 200-600 long-range couplings -0.01 U[0,1] (nnz~46.3M); AFIIM "
+             "eps=0.05; block of 16 RHS b_j=A x*_j, x0=0; FP32 inner block BiCGSTAB (FP32 A and factors, "
+             "to 1e-4) inside per-column FP64 refinement to true rel-res 1e-8",
+        gen="stencil27", N=100, frac=0.05, lo=200, hi=600, seed=11, xseed=12, prec="afiim", args=(0.05,),
+        tol=1e-8, nmax=500, k=16),
+    "c5": dict(
+        desc="serving: Poisson arrivals, 2D 5-point / 3D 7-point convection-diffusion systems (50/50), n "
+             "log-uniform in [1e3, 5e5], log-normal diffusion, Peclet U[0,5], AFIIM eps=0.1 resident; 1-8 RHS "
+             "per request; FP32-inner / FP64-refinement to true rel-res 1e-8 with continuous batching, vs the "
+             "plain FP64 one-at-a-time GPU path on the same requests",
+        prec="afiim", tol=1e-8, nmax=1000),
     "c1": dict(
         desc="2D 5-point convection-diffusion 100^2 (n=10000, nnz=49600), cell Peclet 0.5; AFIIM eps=0.1; "
              "b=A x*, x0=0; plain FP64 BiCGSTAB to rel-res 1e-8",
@@ -57,7 +70,10 @@ CAT_KERNELS = {"A": "spmv_kernel<double, double, cm::EpiAlpha|EpiOmega>",
 
 def build_workload(w):
     import cmamd
-    A = cmamd.gen_convdiff(w["dim"], w["N"], w["pe"], seed=w["seed"])
+    if w.get("gen") == "stencil27":
+        A = cmamd.gen_stencil27(w["N"], w["frac"], w["lo"], w["hi"], seed=w["seed"])
+    else:
+        A = cmamd.gen_convdiff(w["dim"], w["N"], w["pe"], seed=w["seed"])
     xs = cmamd.gen_uniform(w["xseed"], A.n)
     b = cmamd.csr_mv_host(A, xs)
     t0 = time.perf_counter()
@@ -123,6 +139,292 @@ def mixed_leg(sys_, bits, w, A, b, x64, args):
                      f"FP32 vectors, to 1e-4; FP64 refinement of b - A x"}
 
 
+def _pct(v, q):
+    return round(float(np.percentile(v, q)), 3)
+
+
+def bench_block(args, w):
+    """C4: one step = one block solve of k right-hand sides (FP32 inner /
+    FP64 refinement per column).  Baselines on the same inputs: the plain
+    FP64 GPU path one column at a time, and the FP64 block solve."""
+    import cmamd
+    A, _, _, F, t_build = build_workload(w)
+    k = w["k"]
+    XS = np.stack([cmamd.gen_uniform(w["xseed"] + j, A.n) for j in range(k)], axis=1)
+    B = np.stack([cmamd.csr_mv_host(A, XS[:, j]) for j in range(k)], axis=1)
+    print(f"[bench] workload c4: n={A.n} nnz(A)={A.nnz} nnz(G)={F.G.nnz} nnz(H)={F.H.nnz} "
+          f"factors built in {t_build:.1f}s", file=sys.stderr, flush=True)
+    sys_ = cmamd.System(w["prec"], A, F)
+    sys_.enable_mixed(32)
+    steps = max(1, args.steps)
+
+    def timed(fn, reps):
+        lat = []
+        for _ in range(reps):
+            ts = time.perf_counter()
+            out = fn()
+            lat.append(time.perf_counter() - ts)
+        return out, lat
+
+    # mixed block (the measured path)
+    def mixed():
+        sys_.block_set(B)
+        r = sys_.block_solve_mixed(32, tol=w["tol"], inner_tol=1e-4, nmax=w["nmax"])
+        return r, sys_.block_get()
+    for _ in range(args.warmup):
+        mixed()
+    t0 = time.perf_counter()
+    (rmx, Xmx), lat = timed(mixed, steps)
+    elapsed = time.perf_counter() - t0
+    value = k * steps / elapsed
+    # FP64 block and FP64 one column at a time (plain path)
+    def fp64_block():
+        sys_.block_set(B)
+        r = sys_.block_solve(tol=w["tol"], nmax=w["nmax"])
+        return r, sys_.block_get()
+    fp64_block()
+    t0 = time.perf_counter()
+    (r64b, X64b), _ = timed(fp64_block, max(1, steps // 2))
+    v64b = k * max(1, steps // 2) / (time.perf_counter() - t0)
+    cols = min(k, 4)
+    sys_.set_rhs(B[:, 0])
+    sys_.set_x(None)
+    sys_.solve(tol=w["tol"], nmax=w["nmax"])
+    t0 = time.perf_counter()
+    X64 = np.empty((A.n, cols))
+    its = []
+    for j in range(cols):
+        sys_.set_rhs(B[:, j])
+        sys_.set_x(None)
+        _, it, info, _ = sys_.solve(tol=w["tol"], nmax=w["nmax"])
+        X64[:, j] = sys_.get_x()
+        its.append(it)
+    v64 = cols / (time.perf_counter() - t0)
+    # the dominant kernel: FP32 SpMM with A over the 16 columns
+    sys_.block_set(B)
+    us, by = sys_.block_time_spmm("A", 32, reps=20)
+    us_g, by_g = sys_.block_time_spmm("G", 32, reps=20)
+    us_h, by_h = sys_.block_time_spmm("H", 32, reps=20)
+    sys_.close()
+    res = [float(np.linalg.norm(B[:, j] - cmamd.csr_mv_host(A, Xmx[:, j])) / np.linalg.norm(B[:, j]))
+           for j in range(k)]
+    err64 = [float(np.linalg.norm(Xmx[:, j] - X64[:, j]) / np.linalg.norm(X64[:, j])) for j in range(cols)]
+    errx = [float(np.linalg.norm(Xmx[:, j] - XS[:, j]) / np.linalg.norm(XS[:, j])) for j in range(k)]
+    gbs = by / us / 1e3
+    out = {
+        "metric": METRIC, "value": round(value, 3), "unit": "solves/s", "n_gpus": 1, "steps": steps,
+        "warmup": args.warmup, "ms_per_step": round(1e3 * elapsed / steps, 3), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32 inner / f64 refinement", "data": "synthetic",
+        "config": {"workload": w["desc"], "n": A.n, "nnz_A": A.nnz, "nnz_G": F.G.nnz, "nnz_H": F.H.nnz,
+                   "rhs_per_step": k, "tol": w["tol"]},
+        "latency_ms": {"p50": _pct(np.array(lat) * 1e3, 50), "p99": _pct(np.array(lat) * 1e3, 99)},
+        "outer_steps": rmx[1], "inner_passes": rmx[2], "info": [int(i) for i in rmx[3]],
+        "true_rel_residual_max": max(res), "rel_error_vs_fp64_solution_max": max(err64),
+        "forward_error_vs_xstar_max": max(errx),
+        "fp64_one_at_a_time": {"value": round(v64, 3), "unit": "solves/s", "columns_timed": cols,
+                               "iterations": its},
+        "fp64_block": {"value": round(v64b, 3), "unit": "solves/s", "iterations": [int(i) for i in r64b[1]]},
+        "speedup_vs_fp64_one_at_a_time": round(value / v64, 3),
+        "roofline": {"bound": "hbm", "kernel": "spmm_short/spmm_long<float, float, 16> (A, 16 columns)",
+                     "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": None, "bytes_per_launch": by,
+                     "avg_us": round(us, 2), "timing": "20 back-to-back SpMMs between HIP events, separate "
+                                                     "from the timed solves"},
+        "factor_spmm": {"G": {"avg_us": round(us_g, 2), "GBs": round(by_g / us_g / 1e3, 1)},
+                        "H": {"avg_us": round(us_h, 2), "GBs": round(by_h / us_h / 1e3, 1)}},
+    }
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(w, A, B[:, 0], F, args.cpu_seconds)
+    else:
+        out["cpu_baseline"] = None
+    print(json.dumps(out), flush=True)
+
+
+def _c5_systems(nsys, seed):
+    """nsys systems: kind 2D/3D 50/50, n log-uniform in [1e3, 5e5], N =
+    round(n^(1/d)), Peclet U[0,5], log-normal diffusion; AFIIM eps=0.1 built
+    on host threads."""
+    import cmamd
+    from concurrent.futures import ThreadPoolExecutor
+    rng = np.random.default_rng(seed)
+    specs = []
+    for s in range(nsys):
+        d = 2 if rng.random() < 0.5 else 3
+        n = float(np.exp(rng.uniform(np.log(1e3), np.log(5e5))))
+        N = max(4, int(round(n ** (1.0 / d))))
+        specs.append((d, N, float(rng.uniform(0.0, 5.0)), 1000 + s))
+
+    def one(sp):
+        d, N, pe, sd = sp
+        A = cmamd.gen_convdiff(d, N, pe, seed=sd)
+        return A, cmamd.afiim(A, 0.1)
+    with ThreadPoolExecutor(max_workers=min(16, os.cpu_count() or 1)) as ex:
+        return list(ex.map(one, specs))
+
+
+def bench_serve(args, w):
+    """C5: Poisson arrivals of solve requests over resident systems.  The
+    served path (continuous batching, FP32 inner / FP64 refinement) and the
+    plain FP64 one-at-a-time path replay the same request list on the same
+    arrival clock; latency = arrival to solution on the host."""
+    import threading
+    import cmamd
+    rng = np.random.default_rng(args.c5_seed)
+    t0 = time.perf_counter()
+    systems = _c5_systems(args.c5_systems, args.c5_seed)
+    t_build = time.perf_counter() - t0
+    # per system a pool of 8 right-hand sides b = A x*
+    pools = []
+    for s, (A, _) in enumerate(systems):
+        XS = np.stack([cmamd.gen_uniform(50000 + 8 * s + j, A.n) for j in range(8)], axis=1)
+        pools.append((XS, np.stack([cmamd.csr_mv_host(A, XS[:, j]) for j in range(8)], axis=1)))
+    gaps = rng.exponential(1.0 / args.c5_rate, args.c5_requests)
+    arrive = np.cumsum(gaps)
+    reqs = []
+    for i in range(args.c5_requests):
+        s = int(rng.integers(len(systems)))
+        k = int(rng.integers(1, 9))
+        cols = np.sort(rng.choice(8, size=k, replace=False))
+        reqs.append((s, cols))
+    n_tot = sum(A.n for A, _ in systems)
+    print(f"[bench] workload c5: {len(systems)} systems (total n={n_tot}), built in {t_build:.1f}s; "
+          f"{len(reqs)} requests at {args.c5_rate}/s", file=sys.stderr, flush=True)
+    handles = [cmamd.System("afiim", A, F) for A, F in systems]
+
+    def check(i, X):
+        """(worst true rel-res, worst error vs x*) of request i's columns."""
+        s, cols = reqs[i]
+        A = systems[s][0]
+        wr, wx = 0.0, 0.0
+        for j, c in enumerate(cols):
+            b, xs = pools[s][1][:, c], pools[s][0][:, c]
+            wr = max(wr, float(np.linalg.norm(b - cmamd.csr_mv_host(A, X[:, j])) / np.linalg.norm(b)))
+            wx = max(wx, float(np.linalg.norm(X[:, j] - xs) / np.linalg.norm(xs)))
+        return wr, wx
+
+    SAMPLE = 10  # every 10th served solution is kept for the comparison with the FP64 path
+
+    def replay(submit, wait):
+        results = [None] * len(reqs)
+        ids = [None] * len(reqs)
+
+        def waiter():
+            for i in range(len(reqs)):
+                while ids[i] is None:
+                    time.sleep(0.0005)
+                r = wait(ids[i])
+                wr, wx = check(i, r.X)
+                results[i] = (r.latency_ms, r.queue_ms, r.batch_cols, int(np.count_nonzero(r.info)), wr, wx,
+                              r.X if i % SAMPLE == 0 else None)
+        th = threading.Thread(target=waiter)
+        th.start()
+        start = time.perf_counter()
+        for i, (s, cols) in enumerate(reqs):
+            dt = start + arrive[i] - time.perf_counter()
+            if dt > 0:
+                time.sleep(dt)
+            ids[i] = submit(s, pools[s][1][:, cols])
+        th.join()
+        return results, time.perf_counter() - start
+
+    # served: continuous batching, mixed precision
+    srv = cmamd.Server(slots=args.c5_slots, precision=32)
+    sid = [srv.add_system(h) for h in handles]
+    for s in range(len(systems)):  # warm the per-system workspaces once
+        srv.wait(srv.submit(sid[s], pools[s][1][:, :1], tol=w["tol"], nmax=w["nmax"]))
+    served, t_served = replay(lambda s, B: srv.submit(sid[s], B, tol=w["tol"], nmax=w["nmax"]), srv.wait)
+    st = srv.stats()
+    srv.close()
+    # plain FP64 one-at-a-time: each column a single solve on its resident
+    # system, in arrival order, one at a time
+    lock = threading.Lock()
+    queue = []
+    cv = threading.Condition(lock)
+
+    def fp64_worker(out):  # noqa: C901
+        while True:
+            with cv:
+                while not queue:
+                    cv.wait()
+                item = queue.pop(0)
+            if item is None:
+                return
+            i, s, B, t_arr = item
+            h = handles[s]
+            ts = time.perf_counter()
+            X = np.empty_like(B)
+            info = []
+            for j in range(B.shape[1]):
+                h.set_rhs(B[:, j])
+                h.set_x(None)
+                _, _, inf, _ = h.solve(tol=w["tol"], nmax=w["nmax"])
+                X[:, j] = h.get_x()
+                info.append(inf)
+            te = time.perf_counter()
+            out[i] = (X if i % SAMPLE == 0 else None, info, 1e3 * (ts - t_arr), 1e3 * (te - t_arr))
+    base = [None] * len(reqs)
+    th = threading.Thread(target=fp64_worker, args=(base,))
+    th.start()
+    start = time.perf_counter()
+    for i, (s, cols) in enumerate(reqs):
+        dt = start + arrive[i] - time.perf_counter()
+        if dt > 0:
+            time.sleep(dt)
+        with cv:
+            queue.append((i, s, pools[s][1][:, cols], time.perf_counter()))
+            cv.notify()
+    with cv:
+        queue.append(None)
+        cv.notify()
+    th.join()
+    t_base = time.perf_counter() - start
+    ncols = sum(len(c) for _, c in reqs)
+    lat_s = np.array([r[0] for r in served])
+    lat_b = np.array([b[3] for b in base])
+    # replay wall time: first arrival (t = 0) to the last solution on the
+    # host, from the per-request latencies (the harvesting thread's own
+    # checks do not count)
+    t_served = float(np.max(arrive + lat_s / 1e3))
+    t_base = float(np.max(arrive + lat_b / 1e3))
+    # accuracy: true residual and error vs x* of every served column; error
+    # vs the FP64 path's solution on every 10th request
+    worst_res = max(r[4] for r in served)
+    worst_x = max(r[5] for r in served)
+    worst_64 = 0.0
+    for r, b in zip(served, base):
+        if r[6] is None:
+            continue
+        for j in range(r[6].shape[1]):
+            worst_64 = max(worst_64, float(np.linalg.norm(r[6][:, j] - b[0][:, j]) / np.linalg.norm(b[0][:, j])))
+    for h in handles:
+        h.close()
+    value = ncols / t_served
+    v64 = ncols / t_base
+    out = {
+        "metric": METRIC, "value": round(value, 3), "unit": "solves/s", "n_gpus": 1, "steps": len(reqs),
+        "warmup": len(systems), "ms_per_step": round(1e3 * t_served / len(reqs), 3), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32 inner / f64 refinement", "data": "synthetic",
+        "config": {"workload": w["desc"], "systems": len(systems), "requests": len(reqs),
+                   "rate_per_s": args.c5_rate, "rhs_total": ncols, "slots": args.c5_slots, "tol": w["tol"]},
+        "latency_ms": {"p50": _pct(lat_s, 50), "p99": _pct(lat_s, 99)},
+        "queue_ms": {"p50": _pct([r[1] for r in served], 50), "p99": _pct([r[1] for r in served], 99)},
+        "server": st, "mean_batch_columns": round(float(np.mean([r[2] for r in served])), 2),
+        "info_nonzero": int(sum(r[3] for r in served)),
+        "fp64_info_nonzero": int(sum(int(np.count_nonzero(b[1])) for b in base)),
+        "true_rel_residual_max": worst_res, "forward_error_vs_xstar_max": worst_x,
+        "rel_error_vs_fp64_solution_max": worst_64, "fp64_comparison_sample": f"every {SAMPLE}th request",
+        "fp64_one_at_a_time": {"value": round(v64, 3), "unit": "solves/s",
+                               "latency_ms": {"p50": _pct(lat_b, 50), "p99": _pct(lat_b, 99)},
+                               "wall_s": round(t_base, 2)},
+        "speedup_vs_fp64_one_at_a_time": round(value / v64, 3),
+        "wall_s": round(t_served, 2),
+        "roofline": None, "cpu_baseline": None,
+        "note": "solves/s counts right-hand-side columns over the replay (first arrival to last completion); "
+                "with arrivals slower than service it is arrival-bound and latency is the figure of merit",
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -135,6 +437,11 @@ def main():
     ap.add_argument("--kernel-timing", choices=("timed-region", "separate"), default="timed-region",
                     help="bracket the dominant kernel's launches with HIP events inside the timed region "
                          "(default), or time the region without events and time that kernel in extra solves")
+    ap.add_argument("--c5-systems", type=int, default=200)
+    ap.add_argument("--c5-requests", type=int, default=10000)
+    ap.add_argument("--c5-rate", type=float, default=200.0, help="Poisson arrival rate (requests/s)")
+    ap.add_argument("--c5-slots", type=int, default=4)
+    ap.add_argument("--c5-seed", type=int, default=5)
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="run only a plain device copy of 1 GiB (x10), the reference kernel for scaling the "
                          "FETCH_SIZE / WRITE_SIZE hardware counters in rocprofv3 --pmc runs")
@@ -159,6 +466,10 @@ def main():
         print("bench.py: no GPU visible", file=sys.stderr)
         sys.exit(1)
     w = WORKLOADS[args.workload]
+    if args.workload == "c4":
+        return bench_block(args, w)
+    if args.workload == "c5":
+        return bench_serve(args, w)
     A, xs, b, F, t_build = build_workload(w)
     print(f"[bench] workload {args.workload}: n={A.n} nnz(A)={A.nnz} nnz(G)={F.G.nnz} "
           f"nnz(H)={F.H.nnz} factors built in {t_build:.1f}s", file=sys.stderr, flush=True)

@@ -278,7 +278,11 @@ SOLVE_SYMBOLS = [
     "cm_system_destroy", "cm_system_set_rhs", "cm_system_set_x", "cm_system_get_x", "cm_system_solve",
     "cm_system_spmv", "cm_system_apply", "cm_system_nnz", "cm_system_profile", "cm_system_kernel_stats",
     "cm_system_enable_mixed", "cm_system_solve_mixed",
-    "cm_last_error", "cm_device_count",
+    "cm_system_block_set", "cm_system_block_get", "cm_system_block_solve", "cm_system_block_solve_mixed",
+    "cm_system_block_graphs", "cm_system_block_release",
+    "cm_system_block_time_spmm", "cm_last_error", "cm_device_count",
+    "cm_server_create", "cm_server_configure", "cm_server_destroy", "cm_server_add_system", "cm_server_submit", "cm_server_wait",
+    "cm_server_stats",
 ]
 
 
@@ -311,6 +315,30 @@ def _type_solve(lib):
     lib.cm_system_enable_mixed.restype = C.c_int
     lib.cm_system_solve_mixed.argtypes = [vp, C.c_int, C.c_double, C.c_double, C.c_int, C.c_int, PD, PI, PI, PI]
     lib.cm_system_solve_mixed.restype = C.c_int
+    lib.cm_system_block_set.argtypes = [vp, C.c_int, PD, PD]
+    lib.cm_system_block_get.argtypes = [vp, PD]
+    lib.cm_system_block_solve.argtypes = [vp, C.c_double, C.c_int, PD, PI, PI]
+    lib.cm_system_block_solve_mixed.argtypes = [vp, C.c_int, C.c_double, C.c_double, C.c_int, C.c_int, PD, PI,
+                                                PI, PI]
+    lib.cm_system_block_graphs.argtypes = [vp, C.c_int]
+    lib.cm_system_block_release.argtypes = [vp]
+    lib.cm_system_block_time_spmm.argtypes = [vp, C.c_char, C.c_int, C.c_int, PD, PD]
+    lib.cm_system_block_time_spmm.restype = C.c_int
+    for f in ("cm_system_block_set", "cm_system_block_get", "cm_system_block_solve",
+              "cm_system_block_solve_mixed", "cm_system_block_graphs", "cm_system_block_release"):
+        getattr(lib, f).restype = C.c_int
+    PLL = C.POINTER(C.c_longlong)
+    lib.cm_server_create.argtypes = [C.POINTER(vp), C.c_int, C.c_int, C.c_longlong]
+    lib.cm_server_destroy.argtypes = [vp]
+    lib.cm_server_configure.argtypes = [vp, C.c_int, C.c_double, C.c_int]
+    lib.cm_server_configure.restype = C.c_int
+    lib.cm_server_add_system.argtypes = [vp, vp, PI]
+    lib.cm_server_submit.argtypes = [vp, C.c_int, C.c_int, PD, C.c_double, C.c_int, PLL]
+    lib.cm_server_wait.argtypes = [vp, C.c_longlong, PD, PD, PI, PI, PD]
+    lib.cm_server_stats.argtypes = [vp, PLL]
+    for f in ("cm_server_create", "cm_server_destroy", "cm_server_add_system", "cm_server_submit",
+              "cm_server_wait", "cm_server_stats"):
+        getattr(lib, f).restype = C.c_int
     lib.cm_system_profile.argtypes = [vp, C.c_int]
     lib.cm_system_profile.restype = C.c_int
     lib.cm_system_kernel_stats.argtypes = [vp, C.c_int, C.POINTER(C.c_longlong), PD, PD]
@@ -416,6 +444,54 @@ class System:
                                                C.byref(rv), C.byref(o), C.byref(it), C.byref(info)))
         return rv.value, o.value, it.value, info.value
 
+    def block_set(self, B, X0=None):
+        """Resident block of k right-hand sides (B: n x k), initial guesses
+        X0 (n x k) or zeros."""
+        B = np.asfortranarray(np.asarray(B, np.float64).reshape(self.n, -1))
+        self._bk = B.shape[1]
+        if X0 is None:
+            px = None
+        else:
+            X0 = np.asfortranarray(np.asarray(X0, np.float64).reshape(self.n, self._bk))
+            px = _dp(X0)
+        _check(self._lib.cm_system_block_set(self._h, self._bk, _dp(B), px))
+
+    def block_graphs(self, on=True):
+        _check(self._lib.cm_system_block_graphs(self._h, 1 if on else 0))
+
+    def block_time_spmm(self, which="A", bits=64, reps=20):
+        """(average microseconds, compulsory bytes) of the current block's
+        SpMM with matrix `which` at `bits` precision."""
+        us, by = C.c_double(), C.c_double()
+        _check(self._lib.cm_system_block_time_spmm(self._h, which.encode(), bits, reps, C.byref(us),
+                                                   C.byref(by)))
+        return us.value, by.value
+
+    def block_release(self):
+        _check(self._lib.cm_system_block_release(self._h))
+
+    def block_get(self):
+        X = np.empty((self.n, self._bk), np.float64, order="F")
+        _check(self._lib.cm_system_block_get(self._h, _dp(X)))
+        return X
+
+    def block_solve(self, tol=1e-8, nmax=500):
+        """FP64 block solve; per-column (resval, iter, info) arrays."""
+        k = self._bk
+        rv, it, info = np.empty(k), np.empty(k, np.int32), np.empty(k, np.int32)
+        _check(self._lib.cm_system_block_solve(self._h, tol, nmax, _dp(rv), _ip(it), _ip(info)))
+        return rv, it, info
+
+    def block_solve_mixed(self, factor_bits=32, tol=1e-8, inner_tol=1e-4, nmax=500, max_outer=30):
+        """Per-column FP64 refinement around the low-precision inner block
+        solve.  Returns (resval[k], outer steps, inner passes, info[k])."""
+        k = self._bk
+        rv, info = np.empty(k), np.empty(k, np.int32)
+        o, inner = C.c_int(), C.c_int()
+        _check(self._lib.cm_system_block_solve_mixed(self._h, factor_bits, tol, inner_tol, nmax, max_outer,
+                                                     _dp(rv), C.byref(o), C.byref(inner), _ip(info)))
+        return rv, o.value, inner.value, info
+
     def spmv(self, which, x):
         x = np.ascontiguousarray(x, np.float64)
         y = np.empty(self.n, np.float64)
@@ -505,3 +581,75 @@ def acifim_pbicgstab(A: Csr, b, F: Factors, nr=1, mode="E", x0=None, tol=1e-8, n
                                    _dp(uv), _ip(ui), _ip(up), nr, mode.encode(), _dp(x), C.byref(resval),
                                    C.byref(it), verbose, C.byref(info)))
     return SolveResult(x, resval.value, it.value, info.value)
+
+
+@dataclass
+class Served:
+    X: np.ndarray
+    resval: np.ndarray
+    info: np.ndarray
+    outer: int
+    inner: int
+    batch_cols: int
+    queue_ms: float
+    service_ms: float
+    latency_ms: float
+
+
+class Server:
+    """Request loop over resident systems with continuous batching (slots
+    worker streams; requests of one system merged into block solves of up to
+    16 columns; admission by HBM footprint).  precision 64: FP64 BiCGSTAB per
+    column; 32 / 16: FP64 refinement around the FP32 inner solve with FP32 or
+    FP16 factors.  Registered systems must outlive the server and must not be
+    used directly while it runs."""
+
+    def __init__(self, slots=4, precision=32, hbm_budget=0, max_cols=16, inner_tol=1e-4, max_outer=30):
+        self._lib = solve_lib()
+        h = C.c_void_p()
+        _check(self._lib.cm_server_create(C.byref(h), slots, precision, int(hbm_budget)))
+        self._h = h
+        _check(self._lib.cm_server_configure(h, max_cols, inner_tol, max_outer))
+        self._n = {}
+        self._k = {}
+        self._systems = []
+
+    def add_system(self, system: "System") -> int:
+        sid = C.c_int()
+        _check(self._lib.cm_server_add_system(self._h, system._h, C.byref(sid)))
+        self._n[sid.value] = system.n
+        self._systems.append(system)
+        return sid.value
+
+    def submit(self, sys_id, B, tol=1e-8, nmax=500) -> int:
+        n = self._n[sys_id]
+        B = np.asfortranarray(np.asarray(B, np.float64).reshape(n, -1))
+        rid = C.c_longlong()
+        _check(self._lib.cm_server_submit(self._h, sys_id, B.shape[1], _dp(B), tol, nmax, C.byref(rid)))
+        self._k[rid.value] = (n, B.shape[1])
+        return rid.value
+
+    def wait(self, rid) -> Served:
+        n, k = self._k.pop(rid)
+        X = np.empty((n, k), np.float64, order="F")
+        rv, info = np.empty(k), np.empty(k, np.int32)
+        cnt, tms = np.empty(3, np.int32), np.empty(3)
+        _check(self._lib.cm_server_wait(self._h, rid, _dp(X), _dp(rv), _ip(info), _ip(cnt), _dp(tms)))
+        return Served(X, rv, info, int(cnt[0]), int(cnt[1]), int(cnt[2]), float(tms[0]), float(tms[1]),
+                      float(tms[2]))
+
+    def stats(self):
+        s = (C.c_longlong * 6)()
+        _check(self._lib.cm_server_stats(self._h, s))
+        return dict(zip(("batches", "columns", "max_running", "evictions", "bytes", "budget"), list(s)))
+
+    def close(self):
+        if self._h is not None:
+            self._lib.cm_server_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

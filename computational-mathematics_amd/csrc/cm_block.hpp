@@ -1,0 +1,362 @@
+// Multi-right-hand-side (block) BiCGSTAB kernels.
+//
+// KB right-hand sides of one system are solved together.  Every column keeps
+// its own BiCGSTAB recurrence, scalars, exit tests and counters -- the
+// reference loop run once per column -- but the matrix products read A, G
+// and H once for all columns (SpMM), which is where the bandwidth goes.
+//
+// Layout: block vectors are interleaved, element (row i, column j) at
+// i*KB + j, so the KB values a nonzero multiplies sit in one contiguous
+// segment (KB*8 bytes: one 128-byte line for KB = 16 in FP64).
+//
+// SpMM: rows are binned on the host once per matrix.  Short rows (at most
+// SHORT_MAX nonzeros) get KB lanes each (64/KB rows per wave); long rows get
+// a whole wave each, split into 64/KB nonzero groups times KB columns and
+// reduced across the groups with xor shuffles.  Both kernels walk a row list,
+// so a matrix with a few very long rows (the 27-point system with long-range
+// couplings) does not stall the short ones.
+//
+// Per-column state: SolveState[KB]; a column whose done flag is set is
+// frozen (its x, r, p are no longer written), and every kernel returns at
+// once when all columns are done (BlockFlags::all_done).
+#pragma once
+#include "cm_kernels.hpp"
+
+namespace cm {
+
+constexpr int SHORT_MAX = 64;  // longer rows go to the wave-per-row bin
+
+struct BlockFlags {
+    int all_done;
+    int ncols;  // live columns (the rest of the KB slots are padding)
+};
+
+// ------------------------------------------------------------ block epilogues
+// row(r, j, s, acc): row r, column j, row sum s; acc[K] per-column partials.
+
+template <class Y>
+struct BEpiStore {
+    static constexpr int K = 0;
+    static constexpr int XV = 0;
+    Y *y;
+    template <class S>
+    __device__ void row(size_t e, int, S s, double *) const { y[e] = (Y)s; }
+    __device__ void fin(SolveState *, const double *) const {}
+};
+
+template <class Y, class D>
+struct BEpiScale {  // y = d[row] * (A x)
+    static constexpr int K = 0;
+    static constexpr int XV = 0;  // d is an n-vector, counted separately
+    Y *y;
+    const D *d;
+    int kb;
+    template <class S>
+    __device__ void row(size_t e, int, S s, double *) const { y[e] = (Y)d[e / kb] * (Y)s; }
+    __device__ void fin(SolveState *, const double *) const {}
+};
+
+template <class T>
+struct BEpiSetup {  // r = b - A x; r0 = p = r; ||r||^2
+    static constexpr int K = 1;
+    static constexpr int XV = 3;
+    T *r, *r0, *p;
+    const T *b;
+    template <class S>
+    __device__ void row(size_t e, int, S s, double *acc) const
+    {
+        const T v = b[e] + (T)(-1.0) * (T)s;
+        r[e] = v;
+        r0[e] = v;
+        p[e] = v;
+        acc[0] += (double)v * (double)v;
+    }
+    __device__ void fin(SolveState *st, const double *tot) const { FinSetup{}.fin(st, tot); }
+};
+
+template <class T>
+struct BEpiAlpha {
+    static constexpr int K = 1;
+    static constexpr int XV = 1;
+    T *v;
+    const T *r0;
+    template <class S>
+    __device__ void row(size_t e, int, S s, double *acc) const
+    {
+        const T vi = (T)s;
+        v[e] = vi;
+        acc[0] += (double)r0[e] * (double)vi;
+    }
+    __device__ void fin(SolveState *st, const double *tot) const { EpiAlpha<T>{nullptr, nullptr}.fin(st, tot); }
+};
+
+template <class T>
+struct BEpiOmega {
+    static constexpr int K = 2;
+    static constexpr int XV = 1;
+    T *t;
+    const T *s;
+    template <class S>
+    __device__ void row(size_t e, int, S v, double *acc) const
+    {
+        const T ti = (T)v;
+        t[e] = ti;
+        acc[0] += (double)ti * (double)s[e];
+        acc[1] += (double)ti * (double)ti;
+    }
+    __device__ void fin(SolveState *st, const double *tot) const { st->omega = tot[0] / tot[1]; }
+};
+
+struct BEpiResid {  // refinement residual: r = b - A x (FP64), FP32 copy, ||r||^2
+    static constexpr int K = 1;
+    static constexpr int XV = 2;
+    const double *b;
+    float *r32;
+    template <class S>
+    __device__ void row(size_t e, int, S s, double *acc) const
+    {
+        const double v = b[e] - (double)s;
+        r32[e] = (float)v;
+        acc[0] += v * v;
+    }
+    __device__ void fin(SolveState *st, const double *tot) const { st->res_outer = sqrt(tot[0]); }
+};
+
+// Sum over the workgroup of per-column partials.  Lanes j, j+KB, j+2KB, ...
+// of a wave hold column j: xor-shuffle over the row bits, then LDS across
+// waves; thread j (< KB) ends with column j's workgroup total.
+template <int K, int KB>
+__device__ __forceinline__ void block_col_partials(double (&acc)[K > 0 ? K : 1], double *lds /* (BS/WAVE)*K*KB */,
+                                                   double *partials, int slot)
+{
+    if constexpr (K == 0) return;
+    const int lane = threadIdx.x & (WAVE - 1), w = threadIdx.x / WAVE;
+#pragma unroll
+    for (int k = 0; k < K; k++)
+        for (int off = WAVE / 2; off >= KB; off >>= 1) acc[k] += __shfl_xor(acc[k], off, WAVE);
+    __syncthreads();
+    if (lane < KB)
+#pragma unroll
+        for (int k = 0; k < K; k++) lds[(w * K + k) * KB + lane] = acc[k];
+    __syncthreads();
+    if (threadIdx.x < KB)
+#pragma unroll
+        for (int k = 0; k < K; k++) {
+            double s = 0.0;
+            for (int q = 0; q < BS / WAVE; q++) s += lds[(q * K + k) * KB + threadIdx.x];
+            partials[((size_t)slot * K + k) * KB + threadIdx.x] = s;
+        }
+}
+
+// Short rows: KB lanes per row, rows from the list rows[0..cnt).
+template <class V, class T, class Acc, int KB, class Epi>
+__global__ __launch_bounds__(BS) void spmm_short(int cnt, const int *__restrict__ rows, const int *__restrict__ ptr,
+                                                 const int *__restrict__ col, const V *__restrict__ val,
+                                                 const float *__restrict__ rowscale, const T *__restrict__ x,
+                                                 Epi epi, const BlockFlags *fl, double *partials, int slot0)
+{
+    if (fl && fl->all_done) return;
+    __shared__ double red[(BS / WAVE) * (Epi::K > 0 ? Epi::K : 1) * KB];
+    double acc[Epi::K > 0 ? Epi::K : 1];
+#pragma unroll
+    for (int k = 0; k < (Epi::K > 0 ? Epi::K : 1); k++) acc[k] = 0.0;
+    const int j = threadIdx.x & (KB - 1);
+    const int q = xcd_block(blockIdx.x, gridDim.x) * (BS / KB) + threadIdx.x / KB;
+    if (q < cnt) {
+        const int r = rows[q];
+        Acc s = 0;
+        for (int k = ptr[r]; k < ptr[r + 1]; k++)
+            s += load_val<V, Acc>(val, k, false) * (Acc)x[(size_t)col[k] * KB + j];
+        if (rowscale) s *= (Acc)rowscale[r];
+        epi.row((size_t)r * KB + j, j, s, acc);
+    }
+    block_col_partials<Epi::K, KB>(acc, red, partials, slot0 + blockIdx.x);
+}
+
+// Long rows: one wave per row; lane = group g (64/KB of them) x column j.
+template <class V, class T, class Acc, int KB, class Epi>
+__global__ __launch_bounds__(BS) void spmm_long(int cnt, const int *__restrict__ rows, const int *__restrict__ ptr,
+                                                const int *__restrict__ col, const V *__restrict__ val,
+                                                const float *__restrict__ rowscale, const T *__restrict__ x, Epi epi,
+                                                const BlockFlags *fl, double *partials, int slot0)
+{
+    if (fl && fl->all_done) return;
+    __shared__ double red[(BS / WAVE) * (Epi::K > 0 ? Epi::K : 1) * KB];
+    double acc[Epi::K > 0 ? Epi::K : 1];
+#pragma unroll
+    for (int k = 0; k < (Epi::K > 0 ? Epi::K : 1); k++) acc[k] = 0.0;
+    constexpr int G = WAVE / KB;
+    const int lane = threadIdx.x & (WAVE - 1);
+    const int j = lane & (KB - 1), g = lane / KB;
+    const int q = blockIdx.x * (BS / WAVE) + threadIdx.x / WAVE;
+    Acc s = 0;
+    int r = -1;
+    if (q < cnt) {
+        r = rows[q];
+        for (int k = ptr[r] + g; k < ptr[r + 1]; k += G)
+            s += load_val<V, Acc>(val, k, false) * (Acc)x[(size_t)col[k] * KB + j];
+    }
+    for (int off = WAVE / 2; off >= KB; off >>= 1) s += __shfl_xor(s, off, WAVE);
+    if (q < cnt && g == 0) {
+        if (rowscale) s *= (Acc)rowscale[r];
+        epi.row((size_t)r * KB + j, j, s, acc);
+    }
+    block_col_partials<Epi::K, KB>(acc, red, partials, slot0 + blockIdx.x);
+}
+
+// One workgroup: column j's nblk partials of K values in a fixed order, then
+// fin(state[j]) for every live column not done.  Sets all_done.
+template <int K, int KB, class Fin>
+__global__ __launch_bounds__(FBS) void k_block_finalize(int nblk, const double *__restrict__ partials,
+                                                        SolveState *st, BlockFlags *fl, Fin fin, int update_all)
+{
+    if (fl->all_done) return;
+    __shared__ double red[K * FBS / WAVE];
+    for (int j = 0; j < fl->ncols; j++) {
+        double s[K];
+#pragma unroll
+        for (int k = 0; k < K; k++) s[k] = 0.0;
+        for (int b = threadIdx.x; b < nblk; b += FBS)
+#pragma unroll
+            for (int k = 0; k < K; k++) s[k] += partials[((size_t)b * K + k) * KB + j];
+        block_sum<K, FBS>(s, red);
+        if (threadIdx.x == 0 && !st[j].done) fin.fin(&st[j], s);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0 && update_all) {
+        int all = 1;
+        for (int j = 0; j < fl->ncols; j++) all &= (st[j].done != 0);
+        fl->all_done = all;
+    }
+}
+
+// ------------------------------------------------------ block vector kernels
+// Column scalars are read per element from the column's state; done columns
+// are left untouched.
+
+template <class T, int KB>
+__global__ __launch_bounds__(BS) void k_block_s(size_t ne, const T *__restrict__ r, const T *__restrict__ v,
+                                                T *__restrict__ s, const SolveState *st, const BlockFlags *fl)
+{
+    if (fl->all_done) return;
+    __shared__ T na[KB];
+    __shared__ int dn[KB];
+    if (threadIdx.x < KB) {
+        na[threadIdx.x] = (T)(-st[threadIdx.x].alpha);
+        dn[threadIdx.x] = st[threadIdx.x].done;
+    }
+    __syncthreads();
+    for (size_t e = (size_t)blockIdx.x * BS + threadIdx.x; e < ne; e += (size_t)gridDim.x * BS) {
+        const int j = (int)(e & (KB - 1));
+        if (!dn[j]) s[e] = r[e] + na[j] * v[e];
+    }
+}
+
+template <class T, int KB>
+__global__ __launch_bounds__(BS) void k_block_xr(size_t ne, T *__restrict__ x, const T *__restrict__ pc,
+                                                 const T *__restrict__ sc, T *__restrict__ r, const T *__restrict__ s,
+                                                 const T *__restrict__ t, const T *__restrict__ r0,
+                                                 const SolveState *st, const BlockFlags *fl, double *partials)
+{
+    if (fl->all_done) return;
+    __shared__ T al[KB], om[KB];
+    __shared__ int dn[KB];
+    __shared__ double red[(BS / WAVE) * 4 * KB];
+    if (threadIdx.x < KB) {
+        al[threadIdx.x] = (T)st[threadIdx.x].alpha;
+        om[threadIdx.x] = (T)st[threadIdx.x].omega;
+        dn[threadIdx.x] = st[threadIdx.x].done;
+    }
+    __syncthreads();
+    double acc[4] = {0.0, 0.0, 0.0, 0.0};
+    // the grid stride is a multiple of KB, so a thread always sees one column
+    for (size_t e = (size_t)blockIdx.x * BS + threadIdx.x; e < ne; e += (size_t)gridDim.x * BS) {
+        const int j = (int)(e & (KB - 1));
+        if (dn[j]) continue;
+        const T pci = pc[e], xi = x[e], ri = r[e];
+        acc[0] += (double)pci * (double)pci;
+        acc[1] += (double)xi * (double)xi;
+        acc[2] += (double)ri * (double)ri;
+        T xn = xi + al[j] * pci;
+        xn = xn + om[j] * sc[e];
+        x[e] = xn;
+        const T rn = s[e] + (T)(-om[j]) * t[e];
+        r[e] = rn;
+        acc[3] += (double)r0[e] * (double)rn;
+    }
+    block_col_partials<4, KB>(acc, red, partials, blockIdx.x);
+}
+
+template <class T, int KB>
+__global__ __launch_bounds__(BS) void k_block_p(size_t ne, T *__restrict__ p, const T *__restrict__ r,
+                                                const T *__restrict__ v, const SolveState *st, const BlockFlags *fl)
+{
+    if (fl->all_done) return;
+    __shared__ T be[KB], c[KB];
+    __shared__ int dn[KB];
+    if (threadIdx.x < KB) {
+        be[threadIdx.x] = (T)st[threadIdx.x].beta;
+        c[threadIdx.x] = (T)(-st[threadIdx.x].beta * st[threadIdx.x].omega);
+        dn[threadIdx.x] = st[threadIdx.x].done;
+    }
+    __syncthreads();
+    for (size_t e = (size_t)blockIdx.x * BS + threadIdx.x; e < ne; e += (size_t)gridDim.x * BS) {
+        const int j = (int)(e & (KB - 1));
+        if (dn[j]) continue;
+        const T q = c[j] * v[e] + be[j] * p[e];
+        p[e] = q + r[e];
+    }
+}
+
+// Inner setup from x = 0: r = r0 = p = b; per-column ||b||^2 partials.
+template <class T, int KB>
+__global__ __launch_bounds__(BS) void k_block_setup_zero(size_t ne, const T *__restrict__ b, T *__restrict__ x,
+                                                         T *__restrict__ r, T *__restrict__ r0, T *__restrict__ p,
+                                                         double *partials)
+{
+    __shared__ double red[(BS / WAVE) * KB];
+    double acc[1] = {0.0};
+    for (size_t e = (size_t)blockIdx.x * BS + threadIdx.x; e < ne; e += (size_t)gridDim.x * BS) {
+        const T v = b[e];
+        x[e] = (T)0;
+        r[e] = v;
+        r0[e] = v;
+        p[e] = v;
+        acc[0] += (double)v * (double)v;
+    }
+    block_col_partials<1, KB>(acc, red, partials, blockIdx.x);
+}
+
+// Interleave / de-interleave between column-major host blocks (n x ncols)
+// and the [n][KB] device layout; padding columns are zero.
+template <class T, int KB>
+__global__ __launch_bounds__(BS) void k_interleave(int n, int ncols, const double *__restrict__ cm, T *__restrict__ il)
+{
+    const size_t ne = (size_t)n * KB;
+    for (size_t e = (size_t)blockIdx.x * BS + threadIdx.x; e < ne; e += (size_t)gridDim.x * BS) {
+        const int j = (int)(e & (KB - 1));
+        const size_t i = e / KB;
+        il[e] = j < ncols ? (T)cm[(size_t)j * n + i] : (T)0;
+    }
+}
+
+template <int KB>
+__global__ __launch_bounds__(BS) void k_deinterleave(int n, int ncols, const double *__restrict__ il, double *__restrict__ cm)
+{
+    const size_t ne = (size_t)n * KB;
+    for (size_t e = (size_t)blockIdx.x * BS + threadIdx.x; e < ne; e += (size_t)gridDim.x * BS) {
+        const int j = (int)(e & (KB - 1));
+        if (j < ncols) cm[(size_t)j * n + e / KB] = il[e];
+    }
+}
+
+// x += d per live, not-yet-converged column (refinement update).
+template <int KB>
+__global__ __launch_bounds__(BS) void k_block_x_accum(size_t ne, double *__restrict__ x, const float *__restrict__ d,
+                                                      const int *__restrict__ active)
+{
+    for (size_t e = (size_t)blockIdx.x * BS + threadIdx.x; e < ne; e += (size_t)gridDim.x * BS)
+        if (active[e & (KB - 1)]) x[e] += (double)d[e];
+}
+
+}  // namespace cm

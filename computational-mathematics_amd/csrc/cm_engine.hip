@@ -38,6 +38,7 @@
 
 #include "cm_kernels.hpp"
 #include "cm_solve.h"
+#include "cm_block.hpp"
 #include "cm_trsv.hpp"
 
 namespace cm {
@@ -171,6 +172,14 @@ struct DevCsr {
     int *lrows = nullptr;
     std::vector<int> loff;
     bool present() const { return ptr != nullptr; }
+    // device bytes held (index arrays only when owned)
+    size_t bytes() const
+    {
+        if (!present()) return 0;
+        size_t b = (size_t)nnz * sizeof(V) + (rowscale ? (size_t)n * sizeof(float) : 0);
+        if (owns_index) b += (size_t)(n + 1 + nblk + 1) * sizeof(int) + (size_t)nnz * sizeof(int) + (lrows ? (size_t)n * sizeof(int) : 0);
+        return b;
+    }
     void release()
     {
         if (owns_index) {
@@ -456,6 +465,14 @@ struct Krylov {
         if (hst) (void)hipHostFree(hst);
         for (auto &e : ev)
             if (e) (void)hipEventDestroy(e);
+    }
+
+    // device bytes held: matrices, 15 n-vectors, reduction partials, trace
+    size_t bytes() const
+    {
+        return A.bytes() + G.bytes() + H.bytes() + GT.bytes() + L.bytes() + U.bytes() +
+               15 * (size_t)n * sizeof(T) + (size_t)std::max(A.nblk, MAX_RED_BLOCKS) * 4 * sizeof(double) +
+               (size_t)trace_cap_dev * sizeof(double);
     }
 
     bool uses_gh() const { return variant != 2 || mode == 'E' || mode == 'C'; }
@@ -761,6 +778,590 @@ static int create_inner(KI &ki, const K64 &k)
     return CM_OK;
 }
 
+
+// ------------------------------------------------------- block (multi-RHS)
+
+// Row bins of one matrix for the SpMM kernels: rows with at most SHORT_MAX
+// nonzeros (KB lanes each) and longer rows (a wave each).
+struct RowBins {
+    int *shortr = nullptr, *longr = nullptr;
+    int ns = 0, nl = 0;
+    ~RowBins()
+    {
+        (void)hipFree(shortr);
+        (void)hipFree(longr);
+    }
+};
+
+template <class V>
+static int make_bins(RowBins &b, const DevCsr<V> &m, hipStream_t s)
+{
+    if (!m.present()) return CM_OK;
+    std::vector<int> ptr(m.n + 1);
+    CM_HIP(hipMemcpyAsync(ptr.data(), m.ptr, (size_t)(m.n + 1) * sizeof(int), hipMemcpyDeviceToHost, s));
+    CM_HIP(hipStreamSynchronize(s));
+    std::vector<int> sh, lo;
+    for (int i = 0; i < m.n; i++) (ptr[i + 1] - ptr[i] > SHORT_MAX ? lo : sh).push_back(i);
+    b.ns = (int)sh.size();
+    b.nl = (int)lo.size();
+    CM_TRY(dev_upload(&b.shortr, sh.data(), sh.size(), s));
+    CM_TRY(dev_upload(&b.longr, lo.data(), lo.size(), s));
+    CM_HIP(hipStreamSynchronize(s));
+    return CM_OK;
+}
+
+// Block BiCGSTAB over KB interleaved columns, on matrices owned by a Krylov
+// of the same precision (A, G and H or G^T; apply mode E).
+//
+// Iterations are enqueued in chunks (the per-column states say when every
+// column is done).  With graphs on, one pass and GRAPH_PASSES passes are
+// captured once into hipGraphs (kernel nodes only) and a chunk is a few graph
+// launches: the host cost of a pass drops from ~16 kernel launches to a
+// fraction of one graph launch, which is what small systems are bound by.
+constexpr int GRAPH_PASSES = 4;
+
+template <class T, class VA, class VF, int KB>
+struct BlockKrylov {
+    int n = 0, variant = 0, ncols = 0;
+    double eps = 1.11022302462516e-16;
+    hipStream_t stream = nullptr;
+    const DevCsr<VA> *A = nullptr;
+    const DevCsr<VF> *G = nullptr, *Hm = nullptr;
+    const T *idv = nullptr;
+    RowBins bA, bG, bH;
+    T *b = nullptr, *x = nullptr, *r0 = nullptr, *r = nullptr, *p = nullptr, *v = nullptr, *s = nullptr,
+      *t = nullptr, *pc = nullptr, *sc = nullptr, *tm = nullptr;
+    SolveState *st = nullptr, *hst = nullptr;  // KB device states; pinned host copies (3 * KB)
+    BlockFlags *fl = nullptr, *hfl = nullptr;  // device flags; pinned host copies (3)
+    double *partials = nullptr;
+    size_t partial_slots = 0;
+    hipEvent_t ev[2] = {nullptr, nullptr};
+    bool use_graph = true;
+    hipGraphExec_t g1 = nullptr, gN = nullptr;  // one pass, GRAPH_PASSES passes
+
+    ~BlockKrylov()
+    {
+        if (stream) (void)hipStreamSynchronize(stream);
+        if (g1) (void)hipGraphExecDestroy(g1);
+        if (gN) (void)hipGraphExecDestroy(gN);
+        T *bufs[] = {b, x, r0, r, p, v, s, t, pc, sc, tm};
+        for (T *q : bufs) (void)hipFree(q);
+        (void)hipFree(st);
+        (void)hipFree(fl);
+        (void)hipFree(partials);
+        if (hst) (void)hipHostFree(hst);
+        if (hfl) (void)hipHostFree(hfl);
+        for (auto &e : ev)
+            if (e) (void)hipEventDestroy(e);
+    }
+
+    size_t ne() const { return (size_t)n * KB; }
+    int vgrid() const { return (int)std::max<size_t>(1, std::min<size_t>((ne() + BS - 1) / BS, MAX_RED_BLOCKS)); }
+    static int short_grid(const RowBins &bn) { return (bn.ns + (BS / KB) - 1) / (BS / KB); }
+    static int long_grid(const RowBins &bn) { return (bn.nl + (BS / WAVE) - 1) / (BS / WAVE); }
+
+    // device bytes of a block solver of this type for an n-row system whose
+    // three matrices have (short + long) row-bin workgroups `slots`
+    static size_t bytes_for(int n, size_t slots)
+    {
+        const size_t e = (size_t)n * KB;
+        return 11 * e * sizeof(T) + KB * sizeof(SolveState) + sizeof(BlockFlags) +
+               std::max(slots, (size_t)MAX_RED_BLOCKS) * 4 * KB * sizeof(double) + 3 * (size_t)n * sizeof(int);
+    }
+
+    template <class KR>
+    int create(const KR &k, hipStream_t strm)
+    {
+        n = k.n;
+        variant = k.variant;
+        stream = strm;
+        eps = k.eps;
+        A = &k.A;
+        G = &k.G;
+        Hm = k.sym ? &k.GT : &k.H;
+        idv = k.idv;
+        CM_TRY(make_bins(bA, *A, stream));
+        CM_TRY(make_bins(bG, *G, stream));
+        CM_TRY(make_bins(bH, *Hm, stream));
+        T **vecs[] = {&b, &x, &r0, &r, &p, &v, &s, &t, &pc, &sc, &tm};
+        for (T **q : vecs) CM_TRY(dev_alloc(q, ne()));
+        CM_HIP(hipMemsetAsync(x, 0, ne() * sizeof(T), stream));
+        CM_HIP(hipMemsetAsync(b, 0, ne() * sizeof(T), stream));
+        CM_HIP(hipMalloc((void **)&st, KB * sizeof(SolveState)));
+        CM_HIP(hipHostMalloc((void **)&hst, 3 * KB * sizeof(SolveState), hipHostMallocDefault));
+        CM_HIP(hipMalloc((void **)&fl, sizeof(BlockFlags)));
+        CM_HIP(hipHostMalloc((void **)&hfl, 3 * sizeof(BlockFlags), hipHostMallocDefault));
+        size_t slots = (size_t)vgrid();
+        for (const RowBins *bn : {&bA, &bG, &bH})
+            slots = std::max(slots, (size_t)(short_grid(*bn) + long_grid(*bn)));
+        partial_slots = std::max(slots, (size_t)MAX_RED_BLOCKS);
+        CM_HIP(hipMalloc((void **)&partials, partial_slots * 4 * KB * sizeof(double)));
+        for (auto &e : ev) CM_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+        CM_HIP(hipStreamSynchronize(stream));
+        return CM_OK;
+    }
+
+    // y = M x (+ epilogue and, for reducing epilogues, the per-column finalize
+    // on the states fst)
+    template <class V, class Epi>
+    void spmm(const DevCsr<V> &m, const RowBins &bn, const T *xin, Epi epi, SolveState *fst, bool gate = true)
+    {
+        using Acc = typename std::conditional<std::is_same<T, double>::value, double, float>::type;
+        const BlockFlags *gf = gate ? fl : nullptr;
+        const int gs = short_grid(bn), gl = long_grid(bn);
+        if (gs > 0)
+            spmm_short<V, T, Acc, KB, Epi><<<gs, BS, 0, stream>>>(bn.ns, bn.shortr, m.ptr, m.col, m.val, m.rowscale,
+                                                                 xin, epi, gf, partials, 0);
+        if (gl > 0)
+            spmm_long<V, T, Acc, KB, Epi><<<gl, BS, 0, stream>>>(bn.nl, bn.longr, m.ptr, m.col, m.val, m.rowscale,
+                                                                xin, epi, gf, partials, gs);
+        if constexpr (Epi::K > 0)
+            k_block_finalize<Epi::K, KB, Epi><<<1, FBS, 0, stream>>>(gs + gl, partials, fst, fl, epi, gate ? 1 : 0);
+    }
+
+    // Average time (microseconds) of `reps` back-to-back products y = M x of
+    // matrix 'A', 'G' or 'H' (H: the first factor, G^T on the symmetric
+    // branch) over the KB columns, bracketed by events on this stream; bytes =
+    // compulsory traffic of one product (matrix and row lists once, x and y
+    // once per column).
+    template <class V>
+    int time_one(const DevCsr<V> &m, const RowBins &bn, int reps, double *avg_us, double *bytes)
+    {
+        hipEvent_t e0, e1;
+        CM_HIP(hipEventCreate(&e0));
+        CM_HIP(hipEventCreate(&e1));
+        spmm(m, bn, p, BEpiStore<T>{v}, nullptr, false);
+        CM_HIP(hipEventRecord(e0, stream));
+        for (int i = 0; i < reps; i++) spmm(m, bn, p, BEpiStore<T>{v}, nullptr, false);
+        CM_HIP(hipEventRecord(e1, stream));
+        CM_HIP(hipEventSynchronize(e1));
+        float ms = 0.f;
+        CM_HIP(hipEventElapsedTime(&ms, e0, e1));
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        *avg_us = 1e3 * ms / reps;
+        *bytes = (double)m.nnz * (sizeof(V) + sizeof(int)) + 4.0 * (m.n + 1) + 4.0 * (bn.ns + bn.nl) +
+                 2.0 * (double)ne() * sizeof(T) + (m.rowscale ? 4.0 * m.n : 0.0);
+        CM_HIP(hipGetLastError());
+        return CM_OK;
+    }
+
+    int time_spmm(char which, int reps, double *avg_us, double *bytes)
+    {
+        if (which == 'A') return time_one(*A, bA, reps, avg_us, bytes);
+        if (which == 'G') return time_one(*G, bG, reps, avg_us, bytes);
+        if (which == 'H') return time_one(*Hm, bH, reps, avg_us, bytes);
+        return fail(CM_ERR_ARG, "matrix must be 'A', 'G' or 'H'");
+    }
+
+    void enqueue_apply(const T *in, T *out)
+    {
+        spmm(*Hm, bH, in, BEpiScale<T, T>{tm, idv, KB}, st);
+        spmm(*G, bG, tm, BEpiStore<T>{out}, st);
+    }
+
+    void enqueue_iteration()
+    {
+        const int g = vgrid();
+        enqueue_apply(p, pc);
+        spmm(*A, bA, pc, BEpiAlpha<T>{v, r0}, st);
+        k_block_s<T, KB><<<g, BS, 0, stream>>>(ne(), r, v, s, st, fl);
+        enqueue_apply(s, sc);
+        spmm(*A, bA, sc, BEpiOmega<T>{t, s}, st);
+        k_block_xr<T, KB><<<g, BS, 0, stream>>>(ne(), x, pc, sc, r, s, t, r0, st, fl, partials);
+        k_block_finalize<4, KB, FinPass><<<1, FBS, 0, stream>>>(g, partials, st, fl, FinPass{}, 1);
+        k_block_p<T, KB><<<g, BS, 0, stream>>>(ne(), p, r, v, st, fl);
+    }
+
+    int capture(hipGraphExec_t &out, int passes)
+    {
+        hipGraph_t graph = nullptr;
+        CM_HIP(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+        for (int i = 0; i < passes; i++) enqueue_iteration();
+        CM_HIP(hipStreamEndCapture(stream, &graph));
+        const hipError_t err = hipGraphInstantiate(&out, graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        CM_HIP(err);
+        return CM_OK;
+    }
+
+    int enqueue_passes(int size)
+    {
+        if (!use_graph) {
+            for (int i = 0; i < size; i++) enqueue_iteration();
+            CM_HIP(hipGetLastError());
+            return CM_OK;
+        }
+        if (!g1) {
+            CM_TRY(capture(g1, 1));
+            CM_TRY(capture(gN, GRAPH_PASSES));
+        }
+        for (; size >= GRAPH_PASSES; size -= GRAPH_PASSES) CM_HIP(hipGraphLaunch(gN, stream));
+        for (; size > 0; size--) CM_HIP(hipGraphLaunch(g1, stream));
+        return CM_OK;
+    }
+
+    // initial per-column states; columns not in `active` start done
+    int begin(double tol, int NMAX, const int *active)
+    {
+        for (int j = 0; j < KB; j++) {
+            SolveState &q = hst[j];
+            std::memset(&q, 0, sizeof q);
+            q.tol = tol;
+            q.eps = eps;
+            q.nmax = NMAX;
+            q.maxmsteps = max_div_steps(variant, n);
+            q.variant = variant;
+            q.done = (j < ncols && (!active || active[j])) ? 0 : 1;
+        }
+        hfl[0].all_done = 0;
+        hfl[0].ncols = ncols;
+        CM_HIP(hipMemcpyAsync(st, hst, KB * sizeof(SolveState), hipMemcpyHostToDevice, stream));
+        CM_HIP(hipMemcpyAsync(fl, hfl, sizeof(BlockFlags), hipMemcpyHostToDevice, stream));
+        return CM_OK;
+    }
+
+    void setup_from_x() { spmm(*A, bA, x, BEpiSetup<T>{r, r0, p, b}, st); }
+
+    void setup_zero()
+    {
+        const int g = vgrid();
+        k_block_setup_zero<T, KB><<<g, BS, 0, stream>>>(ne(), b, x, r, r0, p, partials);
+        k_block_finalize<1, KB, FinSetup><<<1, FBS, 0, stream>>>(g, partials, st, fl, FinSetup{}, 1);
+    }
+
+    // passes until every column is done; fin receives the final states.
+    // Chunks of passes are enqueued ahead (at most two outstanding, the next
+    // one sized from the residual contraction of the slowest live column).
+    int run(int NMAX, SolveState *fin)
+    {
+        int qs[2], nq = 0, slot = 0, chunk = 4;
+        long long enq = 0;
+        const long long nmax_passes = std::max(NMAX, 0);
+        auto push = [&](int size) -> int {
+            CM_TRY(enqueue_passes(size));
+            CM_HIP(hipMemcpyAsync(&hst[KB * (1 + slot)], st, KB * sizeof(SolveState), hipMemcpyDeviceToHost, stream));
+            CM_HIP(hipMemcpyAsync(&hfl[1 + slot], fl, sizeof(BlockFlags), hipMemcpyDeviceToHost, stream));
+            CM_HIP(hipEventRecord(ev[slot], stream));
+            enq += size;
+            qs[nq++] = slot;
+            slot ^= 1;
+            return CM_OK;
+        };
+        CM_TRY(push((int)std::min<long long>(4, std::max<long long>(nmax_passes, 1))));
+        for (;;) {
+            if (nq < 2 && enq < nmax_passes) CM_TRY(push((int)std::min<long long>(chunk, nmax_passes - enq)));
+            if (nq == 0) return fail(CM_ERR_HIP, "block solve did not reach an exit");
+            const int o = qs[0];
+            qs[0] = qs[1];
+            nq--;
+            CM_HIP(hipEventSynchronize(ev[o]));
+            if (hfl[1 + o].all_done) {
+                std::memcpy(fin, &hst[KB * (1 + o)], KB * sizeof(SolveState));
+                break;
+            }
+            // passes still needed by the slowest live column at its current
+            // contraction rate, assuming the same rate from here on
+            double need = 4.0;
+            for (int j = 0; j < ncols; j++) {
+                const SolveState &q = hst[KB * (1 + o) + j];
+                if (q.done || q.iter < 2 || !(q.rc > 0.0) || !(q.ro > 0.0)) continue;
+                const double rate = std::pow(q.rc / q.ro, 1.0 / q.iter);
+                if (rate < 1.0 && rate > 0.0) need = std::max(need, std::log(q.tol * q.ro / q.rc) / std::log(rate));
+            }
+            const long long ahead = enq - hst[KB * (1 + o)].iter;  // passes already queued beyond this snapshot
+            chunk = (int)std::min<double>(32.0, std::max<double>(2.0, need - (double)std::max<long long>(ahead, 0)));
+        }
+        CM_HIP(hipStreamSynchronize(stream));
+        CM_HIP(hipGetLastError());
+        return CM_OK;
+    }
+};
+
+// Type-erased block solver of one block width.
+struct BlockBase {
+    int kb = 0, ncols = 0;
+    const K32 *k32 = nullptr;  // inner-solve matrices, once mixed precision is enabled
+    const K16 *k16 = nullptr;
+    virtual ~BlockBase() {}
+    virtual int set(int nc, const double *B, const double *X0) = 0;
+    virtual int get(double *X) = 0;
+    virtual int solve64(double tol, int NMAX, double *resval, int *iter, int *info) = 0;
+    virtual int solve_mixed(int bits, double tol, double inner_tol, int NMAX, int max_outer, double *resval,
+                            int *outer, int *inner_total, int *info) = 0;
+    virtual void set_stream(hipStream_t s) = 0;
+    virtual void set_graphs(bool on) = 0;
+    virtual size_t bytes() const = 0;
+    virtual int time_spmm(char which, int bits, int reps, double *avg_us, double *bytes) = 0;
+};
+
+template <int KB>
+struct BlockSolver : BlockBase {
+    hipStream_t stream = nullptr;
+    BlockKrylov<double, double, double, KB> b64;
+    std::unique_ptr<BlockKrylov<float, float, float, KB>> b32;
+    std::unique_ptr<BlockKrylov<float, float, __half, KB>> b16;
+    double *cm = nullptr;      // column-major staging (n x KB)
+    int *active = nullptr;     // device mask of refining columns
+    SolveState *ost = nullptr, *host_ost = nullptr;
+    int n = 0;
+    bool graphs = true;
+    ~BlockSolver()
+    {
+        if (stream) (void)hipStreamSynchronize(stream);
+        (void)hipFree(cm);
+        (void)hipFree(active);
+        (void)hipFree(ost);
+        if (host_ost) (void)hipHostFree(host_ost);
+    }
+
+    int create(const K64 &k)
+    {
+        kb = KB;
+        n = k.n;
+        stream = k.stream;
+        CM_TRY(b64.create(k, stream));
+        CM_TRY(dev_alloc(&cm, (size_t)n * KB));
+        CM_TRY(dev_alloc(&active, (size_t)KB));
+        CM_HIP(hipMalloc((void **)&ost, KB * sizeof(SolveState)));
+        CM_HIP(hipHostMalloc((void **)&host_ost, KB * sizeof(SolveState), hipHostMallocDefault));
+        return CM_OK;
+    }
+
+    size_t bytes() const override
+    {
+        size_t b = decltype(b64)::bytes_for(n, b64.partial_slots) + (size_t)n * KB * sizeof(double);
+        if (b32) b += std::remove_reference<decltype(*b32)>::type::bytes_for(n, b32->partial_slots);
+        if (b16) b += std::remove_reference<decltype(*b16)>::type::bytes_for(n, b16->partial_slots);
+        return b;
+    }
+
+    void set_stream(hipStream_t s) override
+    {
+        stream = s;
+        b64.stream = s;
+        if (b32) b32->stream = s;
+        if (b16) b16->stream = s;
+    }
+
+    void set_graphs(bool on) override
+    {
+        graphs = on;
+        b64.use_graph = on;
+        if (b32) b32->use_graph = on;
+        if (b16) b16->use_graph = on;
+    }
+
+    template <class KI, class BI>
+    int ensure_inner(std::unique_ptr<BI> &bi, const KI *ki)
+    {
+        if (bi) return CM_OK;
+        if (!ki) return fail(CM_ERR_ARG, "mixed precision not enabled (cm_system_enable_mixed)");
+        auto q = std::make_unique<BI>();
+        CM_TRY(q->create(*ki, stream));
+        q->use_graph = graphs;
+        bi = std::move(q);
+        return CM_OK;
+    }
+
+    int set(int nc, const double *B, const double *X0) override
+    {
+        ncols = nc;
+        b64.ncols = nc;
+        const int g = (int)std::min<size_t>(((size_t)n * KB + BS - 1) / BS, 8192);
+        CM_HIP(hipMemcpyAsync(cm, B, (size_t)n * nc * sizeof(double), hipMemcpyHostToDevice, stream));
+        k_interleave<double, KB><<<g, BS, 0, stream>>>(n, nc, cm, b64.b);
+        if (X0) {
+            CM_HIP(hipMemcpyAsync(cm, X0, (size_t)n * nc * sizeof(double), hipMemcpyHostToDevice, stream));
+            k_interleave<double, KB><<<g, BS, 0, stream>>>(n, nc, cm, b64.x);
+        } else {
+            CM_HIP(hipMemsetAsync(b64.x, 0, (size_t)n * KB * sizeof(double), stream));
+        }
+        CM_HIP(hipStreamSynchronize(stream));
+        return CM_OK;
+    }
+
+    int get(double *X) override
+    {
+        const int g = (int)std::min<size_t>(((size_t)n * KB + BS - 1) / BS, 8192);
+        k_deinterleave<KB><<<g, BS, 0, stream>>>(n, ncols, b64.x, cm);
+        CM_HIP(hipMemcpyAsync(X, cm, (size_t)n * ncols * sizeof(double), hipMemcpyDeviceToHost, stream));
+        CM_HIP(hipStreamSynchronize(stream));
+        return CM_OK;
+    }
+
+    int solve64(double tol, int NMAX, double *resval, int *iter, int *info) override
+    {
+        CM_TRY(b64.begin(tol, NMAX, nullptr));
+        b64.setup_from_x();
+        SolveState fin[KB];
+        CM_TRY(b64.run(NMAX, fin));
+        for (int j = 0; j < ncols; j++) {
+            resval[j] = fin[j].rc / fin[j].ro;
+            iter[j] = fin[j].iter;
+            info[j] = fin[j].info;
+        }
+        return CM_OK;
+    }
+
+    // Per-column FP64 refinement (the single-system rules of solve_mixed,
+    // applied to each column): a column leaves the refinement when it
+    // converges, stalls, turns non-finite or reaches max_outer; the inner
+    // block solve runs only the columns still refining.
+    template <class BI>
+    int mixed_with(BI &bi, double tol, double inner_tol, int NMAX, int max_outer, double *resval, int *outer,
+                   int *inner_total, int *info)
+    {
+        bi.ncols = ncols;
+        std::vector<double> res0(ncols, -1.0), prev(ncols, -1.0);
+        std::vector<int> act(KB, 0), fin_info(ncols, -1);
+        int steps = 0, inner = 0;
+        const int g = b64.vgrid();
+        for (int o = 0;; o++) {
+            for (int j = 0; j < KB; j++) {
+                std::memset(&host_ost[j], 0, sizeof(SolveState));
+                host_ost[j].done = (j < ncols && fin_info[j] < 0) ? 0 : 1;
+            }
+            CM_HIP(hipMemcpyAsync(ost, host_ost, KB * sizeof(SolveState), hipMemcpyHostToDevice, stream));
+            b64.hfl[0].all_done = 0;
+            b64.hfl[0].ncols = ncols;
+            CM_HIP(hipMemcpyAsync(b64.fl, b64.hfl, sizeof(BlockFlags), hipMemcpyHostToDevice, stream));
+            b64.spmm(*b64.A, b64.bA, b64.x, BEpiResid{b64.b, bi.b}, ost, false);
+            CM_HIP(hipMemcpyAsync(host_ost, ost, KB * sizeof(SolveState), hipMemcpyDeviceToHost, stream));
+            CM_HIP(hipStreamSynchronize(stream));
+            bool any = false;
+            for (int j = 0; j < ncols; j++) {
+                act[j] = 0;
+                if (fin_info[j] >= 0) continue;
+                const double rn = host_ost[j].res_outer;
+                if (res0[j] < 0.0) res0[j] = rn;
+                resval[j] = res0[j] > 0.0 ? rn / res0[j] : 0.0;
+                if (!std::isfinite(rn)) fin_info[j] = 1;
+                else if (!(rn > tol * res0[j])) fin_info[j] = 0;
+                else if (o >= max_outer) fin_info[j] = 3;
+                else if (prev[j] >= 0.0 && !(rn < prev[j])) fin_info[j] = 2;
+                else {
+                    prev[j] = rn;
+                    act[j] = 1;
+                    any = true;
+                }
+            }
+            steps = o;
+            if (!any) break;
+            CM_TRY(bi.begin(inner_tol, NMAX, act.data()));
+            bi.setup_zero();
+            SolveState fin[KB];
+            CM_TRY(bi.run(NMAX, fin));
+            int most = 0;
+            for (int j = 0; j < ncols; j++)
+                if (act[j]) most = std::max(most, fin[j].ntrace);
+            inner += most;
+            CM_HIP(hipMemcpyAsync(active, act.data(), KB * sizeof(int), hipMemcpyHostToDevice, stream));
+            k_block_x_accum<KB><<<g, BS, 0, stream>>>((size_t)n * KB, b64.x, bi.x, active);
+            CM_HIP(hipGetLastError());
+        }
+        for (int j = 0; j < ncols; j++) info[j] = fin_info[j];
+        if (outer) *outer = steps;
+        if (inner_total) *inner_total = inner;
+        return CM_OK;
+    }
+
+    int time_spmm(char which, int bits, int reps, double *avg_us, double *bytes) override
+    {
+        if (reps < 1) return fail(CM_ERR_ARG, "reps must be positive");
+        if (bits == 64) return b64.time_spmm(which, reps, avg_us, bytes);
+        if (bits == 32) {
+            CM_TRY(ensure_inner(b32, k32));
+            return b32->time_spmm(which, reps, avg_us, bytes);
+        }
+        if (bits == 16) {
+            CM_TRY(ensure_inner(b16, k16));
+            return b16->time_spmm(which, reps, avg_us, bytes);
+        }
+        return fail(CM_ERR_ARG, "bits must be 64, 32 or 16");
+    }
+
+    int solve_mixed(int bits, double tol, double inner_tol, int NMAX, int max_outer, double *resval, int *outer,
+                    int *inner_total, int *info) override
+    {
+        if (bits == 32) {
+            CM_TRY(ensure_inner(b32, k32));
+            return mixed_with(*b32, tol, inner_tol, NMAX, max_outer, resval, outer, inner_total, info);
+        }
+        if (bits == 16) {
+            CM_TRY(ensure_inner(b16, k16));
+            return mixed_with(*b16, tol, inner_tol, NMAX, max_outer, resval, outer, inner_total, info);
+        }
+        return fail(CM_ERR_ARG, "factor_bits must be 32 or 16");
+    }
+};
+
+static int block_width(int k)
+{
+    int kb = 1;
+    while (kb < k) kb *= 2;
+    return kb;
+}
+
+static int block_index(int kb)
+{
+    int i = 0;
+    while ((1 << i) < kb) i++;
+    return i;
+}
+
+// Device bytes a block solver of width kb would take on system k64 (FP64
+// part; the inner solver of the given factor bits adds its own).
+static size_t block_bytes_estimate(const K64 &k64, int kb, int bits)
+{
+    const int n = k64.n;
+    auto slots = [&](int w) {
+        size_t s = 0;
+        for (const auto *m : {&k64.A, k64.sym ? &k64.GT : &k64.H, &k64.G}) {
+            if (!m->present()) continue;
+            // all rows short: n / (BS / w) workgroups bound the bins from above
+            s = std::max(s, (size_t)(m->n + BS / w - 1) / (BS / w) + (size_t)(m->n + BS / WAVE - 1) / (BS / WAVE));
+        }
+        return s;
+    };
+    const size_t sl = slots(kb);
+    auto one = [&](size_t vsz) {
+        return 11 * (size_t)n * kb * vsz + std::max(sl, (size_t)MAX_RED_BLOCKS) * 4 * kb * sizeof(double) +
+               3 * (size_t)n * sizeof(int) + kb * sizeof(SolveState) + sizeof(BlockFlags);
+    };
+    size_t b = one(sizeof(double)) + (size_t)n * kb * sizeof(double);
+    if (bits == 32 || bits == 16) b += one(sizeof(float));
+    return b;
+}
+
+static int make_block(std::unique_ptr<BlockBase> (&cache)[5], BlockBase *&cur, int k, const K64 &k64)
+{
+    if (k < 1 || k > 16) return fail(CM_ERR_ARG, "block width must be 1..16");
+    if (k64.variant == 2 && k64.mode != 'E') return fail(CM_ERR_UNSUPPORTED, "block solves use apply mode E");
+    const int kb = block_width(k);
+    std::unique_ptr<BlockBase> &out = cache[block_index(kb)];
+    if (!out) {
+        auto mk = [&](auto tag) -> int {
+            constexpr int KBV = decltype(tag)::value;
+            auto b = std::make_unique<BlockSolver<KBV>>();
+            CM_TRY(b->create(k64));
+            out = std::move(b);
+            return CM_OK;
+        };
+        int rc;
+        switch (kb) {
+        case 1: rc = mk(std::integral_constant<int, 1>{}); break;
+        case 2: rc = mk(std::integral_constant<int, 2>{}); break;
+        case 4: rc = mk(std::integral_constant<int, 4>{}); break;
+        case 8: rc = mk(std::integral_constant<int, 8>{}); break;
+        default: rc = mk(std::integral_constant<int, 16>{}); break;
+        }
+        if (rc != CM_OK) return rc;
+    }
+    cur = out.get();
+    return CM_OK;
+}
+
 // Results of the last mixed-precision solve.
 struct MixedStats {
     int outer = 0, inner_total = 0, info = 0;
@@ -776,9 +1377,23 @@ struct System {
     SolveState *ost = nullptr;       // device state of the refinement residual
     SolveState *host_ost = nullptr;  // pinned
     MixedStats mixed;
+    // multi-RHS solvers, one per block width 1, 2, 4, 8, 16 (created on
+    // first use; they share the matrices above); blk is the current one
+    std::unique_ptr<BlockBase> blks[5];
+    BlockBase *blk = nullptr;
+    bool block_graphs = true;
+    size_t bytes() const
+    {
+        size_t b = k64->bytes() + (k32 ? k32->bytes() : 0) + (k16 ? k16->bytes() : 0);
+        for (const auto &q : blks)
+            if (q) b += q->bytes();
+        return b;
+    }
     ~System()
     {
         if (stream) (void)hipStreamSynchronize(stream);
+        blk = nullptr;
+        for (auto &q : blks) q.reset();
         k16.reset();
         k32.reset();
         k64.reset();
@@ -1008,12 +1623,14 @@ int cm_system_enable_mixed(cm_system h, int factor_bits)
         CM_HIP(hipHostMalloc((void **)&S.host_ost, sizeof(SolveState), hipHostMallocDefault));
     }
     if (factor_bits == 32 && !S.k32) {
-        S.k32 = std::make_unique<K32>();
-        CM_TRY(create_inner(*S.k32, *S.k64));
+        auto k = std::make_unique<K32>();
+        CM_TRY(create_inner(*k, *S.k64));
+        S.k32 = std::move(k);
     }
     if (factor_bits == 16 && !S.k16) {
-        S.k16 = std::make_unique<K16>();
-        CM_TRY(create_inner(*S.k16, *S.k64));
+        auto k = std::make_unique<K16>();
+        CM_TRY(create_inner(*k, *S.k64));
+        S.k16 = std::move(k);
     }
     return CM_OK;
 }
@@ -1039,6 +1656,63 @@ int cm_system_solve_mixed(cm_system h, int factor_bits, double tol, double inner
     if (inner_total) *inner_total = ms.inner_total;
     if (info) *info = ms.info;
     return CM_OK;
+}
+
+int cm_system_block_set(cm_system h, int k, const double *B, const double *X0)
+{
+    if (!h || !B) return fail(CM_ERR_ARG, "null argument");
+    System &S = h->sys;
+    CM_TRY(make_block(S.blks, S.blk, k, *S.k64));
+    S.blk->k32 = S.k32.get();  // mixed precision may have been enabled since
+    S.blk->k16 = S.k16.get();
+    S.blk->set_graphs(S.block_graphs);
+    return S.blk->set(k, B, X0);
+}
+
+int cm_system_block_graphs(cm_system h, int on)
+{
+    if (!h) return fail(CM_ERR_ARG, "null handle");
+    h->sys.block_graphs = on != 0;
+    for (auto &q : h->sys.blks)
+        if (q) q->set_graphs(on != 0);
+    return CM_OK;
+}
+
+int cm_system_block_time_spmm(cm_system h, char which, int bits, int reps, double *avg_us, double *bytes)
+{
+    if (!h || !avg_us || !bytes || !h->sys.blk) return fail(CM_ERR_ARG, "no block set");
+    h->sys.blk->k32 = h->sys.k32.get();
+    h->sys.blk->k16 = h->sys.k16.get();
+    return h->sys.blk->time_spmm(which, bits, reps, avg_us, bytes);
+}
+
+int cm_system_block_release(cm_system h)
+{
+    if (!h) return fail(CM_ERR_ARG, "null handle");
+    h->sys.blk = nullptr;
+    for (auto &q : h->sys.blks) q.reset();
+    return CM_OK;
+}
+
+int cm_system_block_get(cm_system h, double *X)
+{
+    if (!h || !X || !h->sys.blk) return fail(CM_ERR_ARG, "no block set");
+    return h->sys.blk->get(X);
+}
+
+int cm_system_block_solve(cm_system h, double tol, int NMAX, double *resval, int *iter, int *info)
+{
+    if (!h || !resval || !iter || !info || !h->sys.blk) return fail(CM_ERR_ARG, "no block set");
+    return h->sys.blk->solve64(tol, NMAX, resval, iter, info);
+}
+
+int cm_system_block_solve_mixed(cm_system h, int factor_bits, double tol, double inner_tol, int NMAX, int max_outer,
+                                double *resval, int *outer, int *inner_total, int *info)
+{
+    if (!h || !resval || !info || !h->sys.blk) return fail(CM_ERR_ARG, "no block set");
+    h->sys.blk->k32 = h->sys.k32.get();
+    h->sys.blk->k16 = h->sys.k16.get();
+    return h->sys.blk->solve_mixed(factor_bits, tol, inner_tol, NMAX, max_outer, resval, outer, inner_total, info);
 }
 
 int cm_system_spmv(cm_system h, char which, const double *x, double *y)
@@ -1175,3 +1849,6 @@ int cm_acifim_pbicgstab(int n, double *Av, int *Aj, int *Ai, double *b, double t
 }
 
 }  // extern "C"
+
+#include "cm_serve.h"
+#include "cm_serve.hpp"

@@ -100,6 +100,40 @@ int cm_system_enable_mixed(cm_system sys, int factor_bits);
 int cm_system_solve_mixed(cm_system sys, int factor_bits, double tol, double inner_tol, int NMAX,
                           int max_outer, double *resval, int *outer, int *inner_total, int *info);
 
+/* Multiple right-hand sides of one system, solved together: every column
+ * runs its own BiCGSTAB recurrence with its own scalars, exit tests and
+ * counters -- the reference loop (AFIIM/src/bicgstab.cpp:149-281, called
+ * once per right-hand side) -- while each product with A, G and H reads the
+ * matrix once for all columns.  Apply mode E only (any ACIFIM system built
+ * with another mode returns CM_ERR_UNSUPPORTED).
+ *   block_set:   k columns (1..16), B and X0 column-major n x k (X0 may be
+ *                null: zero initial guesses);
+ *   block_solve: FP64, per-column resval[k], iter[k], info[k] with the
+ *                meaning of cm_system_solve;
+ *   block_solve_mixed: FP64 refinement of each column around the
+ *                low-precision inner block solve (cm_system_enable_mixed
+ *                first); per-column resval[k] and info[k] as in
+ *                cm_system_solve_mixed; outer = refinement steps, inner_total
+ *                = inner passes summed over steps (the longest column);
+ *   block_get:   the k solutions, column-major n x k. */
+int cm_system_block_set(cm_system sys, int k, const double *B, const double *X0);
+int cm_system_block_solve(cm_system sys, double tol, int NMAX, double *resval, int *iter, int *info);
+int cm_system_block_solve_mixed(cm_system sys, int factor_bits, double tol, double inner_tol, int NMAX,
+                                int max_outer, double *resval, int *outer, int *inner_total, int *info);
+int cm_system_block_get(cm_system sys, double *X);
+/* Block passes are replayed from hipGraphs captured on first use (on, the
+ * default) or launched kernel by kernel (off); the results are identical. */
+int cm_system_block_graphs(cm_system sys, int on);
+/* Kernel timing of the current block's SpMM y = M X, M = 'A', 'G' or 'H'
+ * (first factor; G^T on the symmetric PDFIIM branch), in FP64 (bits 64) or
+ * the inner precision (32, 16; cm_system_enable_mixed first): reps
+ * back-to-back products between HIP events on the system's stream; avg_us
+ * per product and its compulsory bytes (matrix and row lists once, X and Y
+ * once per column). */
+int cm_system_block_time_spmm(cm_system sys, char which, int bits, int reps, double *avg_us, double *bytes);
+/* Frees the block workspaces (one per block width used so far). */
+int cm_system_block_release(cm_system sys);
+
 /* Kernel-level entry points (host vectors in/out), for parity checks:
  * which = 'A', 'G', 'H', 'L', 'U' (and 'T' for the G^T copy of a symmetric
  * PDFIIM system): y = M x.  apply: out = M v with the system's preconditioner. */

@@ -1,0 +1,119 @@
+"""GPU tests of the serving loop (continuous batching over resident systems).
+
+Alone vs in the mix: every request is first served alone (submit, wait), then
+all of them are submitted at once, so that requests of one system merge into
+block solves and batches of different systems overlap on the device.  Each
+result must be the same in both runs within the stated tolerance: the same
+info code per column, both solutions at true relative residual <= tol, and
+||x_mix - x_alone|| / ||x_alone|| <= 1e-6 (the two differ only in the
+summation order of the column reductions, which depends on the block width,
+and the refinement then stops at different points below tol).  Results are
+also checked against the FP64 oracle solution.  Admission by HBM footprint is
+exercised with a budget that holds one workspace at a time.
+"""
+import numpy as np
+import pytest
+
+import cmamd
+import oracle
+
+pytestmark = pytest.mark.gpu
+
+XTOL = 1e-6
+
+
+def _rel(a, b):
+    return np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300)
+
+
+def _systems():
+    out = []
+    for d, N, pe, seed in ((2, 60, 0.5, 31), (3, 14, 2.0, 32), (2, 90, 4.0, 33)):
+        A = cmamd.gen_convdiff(d, N, pe, seed=seed)
+        F = cmamd.afiim(A, 0.1)
+        out.append((A, F))
+    return out
+
+
+def _requests(systems, rng):
+    reqs = []
+    for i in range(18):
+        s = int(rng.integers(len(systems)))
+        A = systems[s][0]
+        k = int(rng.integers(1, 9))
+        B = np.stack([cmamd.csr_mv_host(A, cmamd.gen_uniform(1000 + 10 * i + j, A.n)) for j in range(k)], axis=1)
+        reqs.append((s, B))
+    return reqs
+
+
+@pytest.mark.parametrize("precision", [32, 64])
+def test_alone_vs_mix(precision):
+    systems = _systems()
+    handles = [cmamd.System("afiim", A, F) for A, F in systems]
+    srv = cmamd.Server(slots=4, precision=precision)
+    ids = [srv.add_system(h) for h in handles]
+    reqs = _requests(systems, np.random.default_rng(7))
+    alone = []
+    for s, B in reqs:
+        alone.append(srv.wait(srv.submit(ids[s], B, tol=1e-8, nmax=1000)))
+    rids = [srv.submit(ids[s], B, tol=1e-8, nmax=1000) for s, B in reqs]
+    mix = [srv.wait(r) for r in rids]
+    st = srv.stats()
+    srv.close()
+    for h in handles:
+        h.close()
+    # requests merged into shared batches, and batches overlapped
+    assert any(m.batch_cols > B.shape[1] for (_, B), m in zip(reqs, mix))
+    assert st["max_running"] >= 2
+    for (s, B), a, m in zip(reqs, alone, mix):
+        A, F = systems[s]
+        assert np.array_equal(a.info, m.info) and np.all(a.info == 0)
+        for j in range(B.shape[1]):
+            b = B[:, j]
+            for r in (a, m):
+                true_res = np.linalg.norm(b - cmamd.csr_mv_host(A, r.X[:, j])) / np.linalg.norm(b)
+                # precision 64 stops on the recursive residual (the reference's
+                # test), which tracks the true one to a few ulps of ||b||
+                assert true_res <= (1e-8 if precision != 64 else 2e-8)
+            assert _rel(m.X[:, j], a.X[:, j]) <= XTOL
+        assert m.latency_ms >= m.service_ms >= 0.0
+    # against the FP64 oracle, one column per request
+    for (s, B), m in zip(reqs[:6], mix[:6]):
+        A, F = systems[s]
+        xo = oracle.bicgstab("afiim", A, B[:, 0], F, tol=1e-8, nmax=1000)[0]
+        assert _rel(m.X[:, 0], xo) <= 1e-5
+
+
+def test_admission_by_footprint():
+    A0 = cmamd.gen_convdiff(2, 80, 1.0, seed=41)
+    A1 = cmamd.gen_convdiff(2, 80, 1.0, seed=42)
+    F0, F1 = cmamd.afiim(A0, 0.1), cmamd.afiim(A1, 0.1)
+    h0, h1 = cmamd.System("afiim", A0, F0), cmamd.System("afiim", A1, F1)
+    B0 = np.stack([cmamd.csr_mv_host(A0, cmamd.gen_uniform(50 + j, A0.n)) for j in range(4)], axis=1)
+    B1 = np.stack([cmamd.csr_mv_host(A1, cmamd.gen_uniform(60 + j, A1.n)) for j in range(4)], axis=1)
+    # footprint probe: resident bytes, then one 4-column workspace
+    srv = cmamd.Server(slots=2, precision=32)
+    s0, s1 = srv.add_system(h0), srv.add_system(h1)
+    resident = srv.stats()["bytes"]
+    ref0 = srv.wait(srv.submit(s0, B0))
+    ref1 = srv.wait(srv.submit(s1, B1))
+    ws = (srv.stats()["bytes"] - resident) // 2
+    srv.close()
+    h0.block_release()
+    h1.block_release()
+    # room for one workspace: alternating systems must evict, results unchanged
+    srv = cmamd.Server(slots=2, precision=32, hbm_budget=resident + ws + ws // 2)
+    s0, s1 = srv.add_system(h0), srv.add_system(h1)
+    for _ in range(2):
+        r0 = srv.wait(srv.submit(s0, B0))
+        r1 = srv.wait(srv.submit(s1, B1))
+        assert np.array_equal(r0.X, ref0.X) and np.array_equal(r1.X, ref1.X)
+    assert srv.stats()["evictions"] >= 3
+    assert srv.stats()["bytes"] <= resident + ws + ws // 2
+    # a 16-column request needs a workspace four times larger: never fits
+    B16 = np.tile(B0, (1, 4))
+    with pytest.raises(cmamd.EngineError):
+        srv.wait(srv.submit(s0, B16))
+    srv.close()
+    h0.close()
+    h1.close()

@@ -280,7 +280,7 @@ SOLVE_SYMBOLS = [
     "cm_system_enable_mixed", "cm_system_solve_mixed",
     "cm_system_block_set", "cm_system_block_get", "cm_system_block_solve", "cm_system_block_solve_mixed",
     "cm_system_block_graphs", "cm_system_block_release",
-    "cm_system_block_time_spmm", "cm_last_error", "cm_device_count",
+    "cm_system_block_time_spmm", "cm_system_autotune", "cm_system_block_autotune", "cm_last_error", "cm_device_count",
     "cm_server_create", "cm_server_configure", "cm_server_destroy", "cm_server_add_system", "cm_server_submit", "cm_server_wait",
     "cm_server_stats",
 ]
@@ -324,6 +324,10 @@ def _type_solve(lib):
     lib.cm_system_block_release.argtypes = [vp]
     lib.cm_system_block_time_spmm.argtypes = [vp, C.c_char, C.c_int, C.c_int, PD, PD]
     lib.cm_system_block_time_spmm.restype = C.c_int
+    lib.cm_system_autotune.argtypes = [vp, C.c_int, PI]
+    lib.cm_system_autotune.restype = C.c_int
+    lib.cm_system_block_autotune.argtypes = [vp, C.c_int, C.c_int, PI]
+    lib.cm_system_block_autotune.restype = C.c_int
     for f in ("cm_system_block_set", "cm_system_block_get", "cm_system_block_solve",
               "cm_system_block_solve_mixed", "cm_system_block_graphs", "cm_system_block_release"):
         getattr(lib, f).restype = C.c_int
@@ -466,6 +470,18 @@ class System:
         _check(self._lib.cm_system_block_time_spmm(self._h, which.encode(), bits, reps, C.byref(us),
                                                    C.byref(by)))
         return us.value, by.value
+
+    def autotune(self, reps=10):
+        """Tile per resident matrix (A, G, H, GT, L, U; 0 = absent)."""
+        t = np.zeros(6, np.int32)
+        _check(self._lib.cm_system_autotune(self._h, reps, _ip(t)))
+        return dict(zip(("A", "G", "H", "GT", "L", "U"), (int(v) for v in t)))
+
+    def block_autotune(self, bits=64, reps=10):
+        """Short-row limit of the current block's SpMM bins for A, G, H."""
+        t = np.zeros(3, np.int32)
+        _check(self._lib.cm_system_block_autotune(self._h, bits, reps, _ip(t)))
+        return dict(zip(("A", "G", "H"), (int(v) for v in t)))
 
     def block_release(self):
         _check(self._lib.cm_system_block_release(self._h))

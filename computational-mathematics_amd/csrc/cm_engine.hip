@@ -99,9 +99,9 @@ __global__ __launch_bounds__(BS) void k_vmul(int n, const T *__restrict__ a, con
 
 // ------------------------------------------------------------ host helpers
 
-// Row blocks for the adaptive SpMV: consecutive rows with at most TILE
+// Row blocks for the adaptive SpMV: consecutive rows with at most `tile`
 // nonzeros and ROWCAP rows, or a single longer row.
-static std::vector<int> make_row_blocks(int n, const int *ptr)
+static std::vector<int> make_row_blocks(int n, const int *ptr, int tile = TILE)
 {
     std::vector<int> b;
     b.push_back(0);
@@ -109,14 +109,14 @@ static std::vector<int> make_row_blocks(int n, const int *ptr)
     long long nz = 0;
     for (int r = 0; r < n; r++) {
         const int len = ptr[r + 1] - ptr[r];
-        if (rows > 0 && (nz + len > TILE || rows >= ROWCAP)) {
+        if (rows > 0 && (nz + len > tile || rows >= ROWCAP)) {
             b.push_back(r);
             rows = 0;
             nz = 0;
         }
         rows++;
         nz += len;
-        if (len > TILE) {
+        if (len > tile) {
             b.push_back(r + 1);
             rows = 0;
             nz = 0;
@@ -164,6 +164,7 @@ struct DevCsr {
     int n = 0;
     long long nnz = 0;
     int nblk = 0;
+    int tile = TILE;  // staged nonzeros per SpMV workgroup the row blocks were cut for
     bool owns_index = true;
     int *ptr = nullptr, *col = nullptr, *blk = nullptr;
     V *val = nullptr;
@@ -235,6 +236,7 @@ static int cast_csr(DevCsr<V> &dst, const DevCsr<double> &src, hipStream_t s)
     dst.n = src.n;
     dst.nnz = src.nnz;
     dst.nblk = src.nblk;
+    dst.tile = src.tile;
     dst.owns_index = false;
     dst.ptr = src.ptr;
     dst.col = src.col;
@@ -449,6 +451,7 @@ struct Krylov {
     SolveState *st = nullptr;   // device
     SolveState *hst = nullptr;  // pinned host, 3 slots (init + 2 polling slots)
     double *partials = nullptr;
+    size_t partials_cap = 0;  // doubles
     double *trace = nullptr;
     int trace_cap_dev = 0;
     hipEvent_t ev[2] = {nullptr, nullptr};
@@ -489,7 +492,21 @@ struct Krylov {
         CM_HIP(hipHostMalloc((void **)&hst, 3 * sizeof(SolveState), hipHostMallocDefault));
         const int maxblk = std::max(nblk_a, MAX_RED_BLOCKS);
         CM_HIP(hipMalloc((void **)&partials, (size_t)maxblk * 4 * sizeof(double)));
+        partials_cap = (size_t)maxblk * 4;
         for (auto &e : ev) CM_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+        return CM_OK;
+    }
+
+    // the reduction partials must hold K <= 4 values per SpMV workgroup
+    int ensure_partials(int nblk)
+    {
+        const size_t need = (size_t)std::max(nblk, MAX_RED_BLOCKS) * 4;
+        if (need <= partials_cap) return CM_OK;
+        CM_HIP(hipStreamSynchronize(stream));
+        (void)hipFree(partials);
+        partials = nullptr;
+        CM_HIP(hipMalloc((void **)&partials, need * sizeof(double)));
+        partials_cap = need;
         return CM_OK;
     }
 
@@ -535,9 +552,17 @@ struct Krylov {
         using Acc = typename std::conditional<std::is_same<T, double>::value, double, float>::type;
         // fst: a separate state (refinement residual) instead of the solve's
         SolveState *kst = fst ? fst : st;
+        SolveState *gate = use_state ? kst : nullptr;
         timed(cat, spmv_bytes<V, Epi>(m), [&] {
-            spmv_kernel<V, T, Acc, Epi><<<m.nblk, BS, 0, stream>>>(m.ptr, m.col, m.val, m.rowscale, m.blk, xin,
-                                                                  epi, use_state ? kst : nullptr, partials);
+            if (m.tile == 1024)
+                spmv_kernel<1024, V, T, Acc, Epi><<<m.nblk, BS, 0, stream>>>(m.ptr, m.col, m.val, m.rowscale,
+                                                                            m.blk, xin, epi, gate, partials);
+            else if (m.tile == 4096)
+                spmv_kernel<4096, V, T, Acc, Epi><<<m.nblk, BS, 0, stream>>>(m.ptr, m.col, m.val, m.rowscale,
+                                                                            m.blk, xin, epi, gate, partials);
+            else
+                spmv_kernel<2048, V, T, Acc, Epi><<<m.nblk, BS, 0, stream>>>(m.ptr, m.col, m.val, m.rowscale,
+                                                                            m.blk, xin, epi, gate, partials);
         });
         if constexpr (Epi::K > 0)
             timed(KC_FIN, 8.0 * Epi::K * m.nblk,
@@ -779,6 +804,38 @@ static int create_inner(KI &ki, const K64 &k)
 }
 
 
+// ------------------------------------------------------------- autotuner
+//
+// Row blocks of a resident FP64 matrix re-cut for `tile` staged nonzeros per
+// SpMV workgroup.  The row pointers are read back from the device.
+static int retile(DevCsr<double> &m, int tile, hipStream_t s)
+{
+    if (!m.present() || m.tile == tile) return CM_OK;
+    std::vector<int> ptr(m.n + 1);
+    CM_HIP(hipMemcpyAsync(ptr.data(), m.ptr, (size_t)(m.n + 1) * sizeof(int), hipMemcpyDeviceToHost, s));
+    CM_HIP(hipStreamSynchronize(s));
+    std::vector<int> blk = make_row_blocks(m.n, ptr.data(), tile);
+    int *nb = nullptr;
+    CM_TRY(dev_upload(&nb, blk.data(), blk.size(), s));
+    CM_HIP(hipStreamSynchronize(s));
+    (void)hipFree(m.blk);
+    m.blk = nb;
+    m.nblk = (int)blk.size() - 1;
+    m.tile = tile;
+    return CM_OK;
+}
+
+// A low-precision copy shares the FP64 matrix's index arrays: follow its
+// row blocks.
+template <class V>
+static void follow_blocks(DevCsr<V> &dst, const DevCsr<double> &src)
+{
+    if (!dst.present()) return;
+    dst.blk = src.blk;
+    dst.nblk = src.nblk;
+    dst.tile = src.tile;
+}
+
 // ------------------------------------------------------- block (multi-RHS)
 
 // Row bins of one matrix for the SpMM kernels: rows with at most SHORT_MAX
@@ -786,6 +843,7 @@ static int create_inner(KI &ki, const K64 &k)
 struct RowBins {
     int *shortr = nullptr, *longr = nullptr;
     int ns = 0, nl = 0;
+    int short_max = SHORT_MAX;  // rows longer than this go to the wave-per-row kernel
     ~RowBins()
     {
         (void)hipFree(shortr);
@@ -794,14 +852,18 @@ struct RowBins {
 };
 
 template <class V>
-static int make_bins(RowBins &b, const DevCsr<V> &m, hipStream_t s)
+static int make_bins(RowBins &b, const DevCsr<V> &m, hipStream_t s, int short_max = SHORT_MAX)
 {
     if (!m.present()) return CM_OK;
     std::vector<int> ptr(m.n + 1);
     CM_HIP(hipMemcpyAsync(ptr.data(), m.ptr, (size_t)(m.n + 1) * sizeof(int), hipMemcpyDeviceToHost, s));
     CM_HIP(hipStreamSynchronize(s));
     std::vector<int> sh, lo;
-    for (int i = 0; i < m.n; i++) (ptr[i + 1] - ptr[i] > SHORT_MAX ? lo : sh).push_back(i);
+    for (int i = 0; i < m.n; i++) (ptr[i + 1] - ptr[i] > short_max ? lo : sh).push_back(i);
+    (void)hipFree(b.shortr);
+    (void)hipFree(b.longr);
+    b.shortr = b.longr = nullptr;
+    b.short_max = short_max;
     b.ns = (int)sh.size();
     b.nl = (int)lo.size();
     CM_TRY(dev_upload(&b.shortr, sh.data(), sh.size(), s));
@@ -943,6 +1005,57 @@ struct BlockKrylov {
         *bytes = (double)m.nnz * (sizeof(V) + sizeof(int)) + 4.0 * (m.n + 1) + 4.0 * (bn.ns + bn.nl) +
                  2.0 * (double)ne() * sizeof(T) + (m.rowscale ? 4.0 * m.n : 0.0);
         CM_HIP(hipGetLastError());
+        return CM_OK;
+    }
+
+    // Autotuner of the row binning: for each matrix, the short-row limit
+    // (16, 32, 64 or 128 nonzeros) whose SpMM over the KB columns is fastest
+    // (reps products per candidate, events on this stream).  Captured graphs
+    // are dropped (their launches carry the old bins).
+    template <class V>
+    int tune_one(const DevCsr<V> &m, RowBins &bn, int reps, int *chosen)
+    {
+        *chosen = 0;
+        if (!m.present()) return CM_OK;
+        double best = 1e300;
+        int best_t = SHORT_MAX;
+        for (int thr : {16, 32, 64, 128}) {
+            CM_TRY(make_bins(bn, m, stream, thr));
+            CM_TRY(fit_partials());
+            double us = 0.0, by = 0.0;
+            CM_TRY(time_one(m, bn, reps, &us, &by));
+            if (us < best) {
+                best = us;
+                best_t = thr;
+            }
+        }
+        CM_TRY(make_bins(bn, m, stream, best_t));
+        *chosen = best_t;
+        return CM_OK;
+    }
+
+    int fit_partials()
+    {
+        size_t slots = (size_t)vgrid();
+        for (const RowBins *bn : {&bA, &bG, &bH})
+            slots = std::max(slots, (size_t)(short_grid(*bn) + long_grid(*bn)));
+        if (slots <= partial_slots) return CM_OK;
+        CM_HIP(hipStreamSynchronize(stream));
+        (void)hipFree(partials);
+        partials = nullptr;
+        CM_HIP(hipMalloc((void **)&partials, slots * 4 * KB * sizeof(double)));
+        partial_slots = slots;
+        return CM_OK;
+    }
+
+    int tune(int reps, int *chosen /* A, G, H */)
+    {
+        if (g1) (void)hipGraphExecDestroy(g1);
+        if (gN) (void)hipGraphExecDestroy(gN);
+        g1 = gN = nullptr;
+        CM_TRY(tune_one(*A, bA, reps, &chosen[0]));
+        CM_TRY(tune_one(*G, bG, reps, &chosen[1]));
+        CM_TRY(tune_one(*Hm, bH, reps, &chosen[2]));
         return CM_OK;
     }
 
@@ -1093,6 +1206,7 @@ struct BlockBase {
     virtual void set_graphs(bool on) = 0;
     virtual size_t bytes() const = 0;
     virtual int time_spmm(char which, int bits, int reps, double *avg_us, double *bytes) = 0;
+    virtual int tune(int bits, int reps, int *chosen) = 0;
 };
 
 template <int KB>
@@ -1264,6 +1378,21 @@ struct BlockSolver : BlockBase {
         if (outer) *outer = steps;
         if (inner_total) *inner_total = inner;
         return CM_OK;
+    }
+
+    int tune(int bits, int reps, int *chosen) override
+    {
+        if (reps < 1) return fail(CM_ERR_ARG, "reps must be positive");
+        if (bits == 64) return b64.tune(reps, chosen);
+        if (bits == 32) {
+            CM_TRY(ensure_inner(b32, k32));
+            return b32->tune(reps, chosen);
+        }
+        if (bits == 16) {
+            CM_TRY(ensure_inner(b16, k16));
+            return b16->tune(reps, chosen);
+        }
+        return fail(CM_ERR_ARG, "bits must be 64, 32 or 16");
     }
 
     int time_spmm(char which, int bits, int reps, double *avg_us, double *bytes) override
@@ -1684,6 +1813,67 @@ int cm_system_block_time_spmm(cm_system h, char which, int bits, int reps, doubl
     h->sys.blk->k32 = h->sys.k32.get();
     h->sys.blk->k16 = h->sys.k16.get();
     return h->sys.blk->time_spmm(which, bits, reps, avg_us, bytes);
+}
+
+int cm_system_autotune(cm_system h, int reps, int *tiles)
+{
+    if (!h || reps < 1) return fail(CM_ERR_ARG, "bad argument");
+    System &S = h->sys;
+    K64 &k = *S.k64;
+    DevCsr<double> *ms[] = {&k.A, &k.G, &k.H, &k.GT, &k.L, &k.U};
+    hipEvent_t e0, e1;
+    CM_HIP(hipEventCreate(&e0));
+    CM_HIP(hipEventCreate(&e1));
+    int maxblk = 0;
+    for (int q = 0; q < 6; q++) {
+        DevCsr<double> &m = *ms[q];
+        if (tiles) tiles[q] = 0;
+        if (!m.present()) continue;
+        double best = 1e300;
+        int best_t = TILE;
+        for (int tile : {1024, 2048, 4096}) {
+            CM_TRY(retile(m, tile, k.stream));
+            k.spmv(m, k.x, EpiStore<double>{k.t}, false, KC_A);  // warm
+            CM_HIP(hipEventRecord(e0, k.stream));
+            for (int i = 0; i < reps; i++) k.spmv(m, k.x, EpiStore<double>{k.t}, false, KC_A);
+            CM_HIP(hipEventRecord(e1, k.stream));
+            CM_HIP(hipEventSynchronize(e1));
+            float ms_ = 0.f;
+            CM_HIP(hipEventElapsedTime(&ms_, e0, e1));
+            if (ms_ < best) {
+                best = ms_;
+                best_t = tile;
+            }
+        }
+        CM_TRY(retile(m, best_t, k.stream));
+        maxblk = std::max(maxblk, m.nblk);
+        if (tiles) tiles[q] = best_t;
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    CM_TRY(k.ensure_partials(maxblk));
+    if (S.k32) {
+        K32 &c = *S.k32;
+        follow_blocks(c.A, k.A), follow_blocks(c.G, k.G), follow_blocks(c.H, k.H);
+        follow_blocks(c.GT, k.GT), follow_blocks(c.L, k.L), follow_blocks(c.U, k.U);
+        CM_TRY(c.ensure_partials(maxblk));
+    }
+    if (S.k16) {
+        K16 &c = *S.k16;
+        follow_blocks(c.A, k.A), follow_blocks(c.G, k.G), follow_blocks(c.H, k.H);
+        follow_blocks(c.GT, k.GT), follow_blocks(c.L, k.L), follow_blocks(c.U, k.U);
+        CM_TRY(c.ensure_partials(maxblk));
+    }
+    CM_HIP(hipGetLastError());
+    return CM_OK;
+}
+
+int cm_system_block_autotune(cm_system h, int bits, int reps, int *short_max)
+{
+    if (!h || !short_max || !h->sys.blk) return fail(CM_ERR_ARG, "no block set");
+    h->sys.blk->k32 = h->sys.k32.get();
+    h->sys.blk->k16 = h->sys.k16.get();
+    return h->sys.blk->tune(bits, reps, short_max);
 }
 
 int cm_system_block_release(cm_system h)

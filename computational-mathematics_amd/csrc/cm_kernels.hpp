@@ -37,7 +37,7 @@
 namespace cm {
 
 constexpr int BS = 256;       // threads per workgroup (4 waves of 64)
-constexpr int TILE = 2048;    // nonzeros staged per SpMV workgroup
+constexpr int TILE = 2048;    // default nonzeros staged per SpMV workgroup (tunable: 1024, 2048, 4096)
 constexpr int ROWCAP = 1024;  // rows per SpMV workgroup
 constexpr int WAVE = 64;
 constexpr int MAX_RED_BLOCKS = 2048;  // workgroups of the reducing vector kernels
@@ -333,8 +333,9 @@ __device__ __forceinline__ double load_val<double, double>(const double *p, int 
 
 // y = A x (+ epilogue).  A: CSR ptr/col/val (rowscale: optional per-row
 // factor applied to the row sum, for FP16 values stored scaled); blk:
-// row-block starts (gridDim.x + 1).  Products and row sums in Acc.
-template <class V, class X, class Acc, class Epi>
+// row-block starts (gridDim.x + 1), cut for TL staged nonzeros per block.
+// Products and row sums in Acc.
+template <int TL, class V, class X, class Acc, class Epi>
 __global__ __launch_bounds__(BS) void spmv_kernel(const int *__restrict__ ptr,
                                                   const int *__restrict__ col,
                                                   const V *__restrict__ val,
@@ -344,7 +345,7 @@ __global__ __launch_bounds__(BS) void spmv_kernel(const int *__restrict__ ptr,
                                                   SolveState *st, double *partials)
 {
     if (st && st->done) return;
-    __shared__ Acc prod[TILE];
+    __shared__ Acc prod[TL];
     __shared__ double red[(Epi::K > 0 ? Epi::K : 1) * BS / WAVE];
     double acc[Epi::K > 0 ? Epi::K : 1];
 #pragma unroll
@@ -357,7 +358,7 @@ __global__ __launch_bounds__(BS) void spmv_kernel(const int *__restrict__ ptr,
     const int cnt = re - rs;
     const int nr = r1 - r0;
 
-    if (cnt > TILE) {
+    if (cnt > TL) {
         // one long row: the whole workgroup strides over it
         double s = 0.0;
         for (int k = rs + tid; k < re; k += BS) s += (double)(load_val<V, Acc>(val, k, false) * (Acc)x[col[k]]);
@@ -369,10 +370,10 @@ __global__ __launch_bounds__(BS) void spmv_kernel(const int *__restrict__ ptr,
             epi.row(r0, rsum, acc);
         }
     } else {
-        int cidx[TILE / BS];
-        Acc cv[TILE / BS];
+        int cidx[TL / BS];
+        Acc cv[TL / BS];
 #pragma unroll
-        for (int u = 0; u < TILE / BS; u++) {
+        for (int u = 0; u < TL / BS; u++) {
             const int k = tid + u * BS;
             if (k < cnt) {  // streamed once: nontemporal, keeps x in the caches
                 cidx[u] = __builtin_nontemporal_load(&col[rs + k]);
@@ -380,7 +381,7 @@ __global__ __launch_bounds__(BS) void spmv_kernel(const int *__restrict__ ptr,
             }
         }
 #pragma unroll
-        for (int u = 0; u < TILE / BS; u++) {
+        for (int u = 0; u < TL / BS; u++) {
             const int k = tid + u * BS;
             if (k < cnt) prod[k] = cv[u] * (Acc)x[cidx[u]];
         }

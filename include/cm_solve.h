@@ -131,6 +131,17 @@ int cm_system_block_graphs(cm_system sys, int on);
  * per product and its compulsory bytes (matrix and row lists once, X and Y
  * once per column). */
 int cm_system_block_time_spmm(cm_system sys, char which, int bits, int reps, double *avg_us, double *bytes);
+/* Autotuner of the single-RHS SpMV: for each resident matrix (A, G, H,
+ * G^T, L, U), times reps FP64 products with row blocks cut for 1024, 2048
+ * and 4096 staged nonzeros per workgroup and keeps the fastest (the FP32 /
+ * FP16 copies follow).  tiles[6]: the choices, 0 for an absent matrix.
+ * Results of later solves are unchanged up to the summation order of the
+ * row blocks' partial dot products. */
+int cm_system_autotune(cm_system sys, int reps, int *tiles);
+/* Autotuner of the current block's SpMM row binning (bits 64, 32 or 16):
+ * for A, G and H the short-row limit among 16, 32, 64, 128 nonzeros that
+ * times fastest; short_max[3] receives the choices. */
+int cm_system_block_autotune(cm_system sys, int bits, int reps, int *short_max);
 /* Frees the block workspaces (one per block width used so far). */
 int cm_system_block_release(cm_system sys);
 

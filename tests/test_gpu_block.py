@@ -221,3 +221,25 @@ def test_block_graph_replays_match_eager():
         assert np.array_equal(got[3][0], ref[3][0]) and got[3][1:3] == ref[3][1:3]
         assert np.array_equal(got[3][3], ref[3][3])
     sys_.close()
+
+
+def test_block_autotuned_bins_keep_parity():
+    A = cmamd.gen_stencil27(14, 0.05, 200, 600, seed=9)
+    F = cmamd.afiim(A, 0.1)
+    B = _rhs(A, 8, 220)
+    sys_ = cmamd.System("afiim", A, F)
+    sys_.enable_mixed(32)
+    sys_.block_set(B)
+    for bits in (64, 32):
+        chosen = sys_.block_autotune(bits, reps=5)
+        assert all(v in (16, 32, 64, 128) for v in chosen.values())
+    rv, it, info = sys_.block_solve(tol=1e-8, nmax=500)
+    X = sys_.block_get()
+    _check_columns("afiim", A, F, B, X, rv, it, info, 500)
+    sys_.block_set(B)
+    rv, outer, inner, info = sys_.block_solve_mixed(32, tol=1e-8, inner_tol=1e-4, nmax=500)
+    X = sys_.block_get()
+    sys_.close()
+    assert np.all(info == 0)
+    for j in range(B.shape[1]):
+        assert np.linalg.norm(B[:, j] - cmamd.csr_mv_host(A, X[:, j])) / np.linalg.norm(B[:, j]) <= 1e-8

@@ -232,3 +232,32 @@ def test_mixed_precision_refinement(bits, case):
     # exact solution in the residual sense
     assert _rel(x, xo) <= 1e3 * tol
     assert 1 <= outer <= 6
+
+
+def test_autotuned_tiles_keep_parity():
+    """The tuner re-cuts the SpMV row blocks per matrix; solves afterwards
+    meet the same bars against the oracle (FP64 and mixed)."""
+    A = cmamd.gen_convdiff(3, 24, 1.0, seed=21)
+    b = cmamd.csr_mv_host(A, cmamd.gen_uniform(22, A.n))
+    F = cmamd.pdfiim(A, 2, 2, 1, 0.05, -1, "I", "B")
+    sys_ = cmamd.System("pdfiim", A, F)
+    sys_.enable_mixed(16)
+    tiles = sys_.autotune(reps=5)
+    assert tiles["A"] in (1024, 2048, 4096) and tiles["G"] in (1024, 2048, 4096)
+    assert tiles["L"] == 0 and tiles["U"] == 0
+    x = cmamd.gen_uniform(23, A.n)
+    assert _rel(sys_.spmv("A", x), oracle.csr_mv(A, x)) <= SPMV_TOL
+    assert _rel(sys_.apply(x), oracle.apply(F, x)) <= SPMV_TOL
+    sys_.set_rhs(b)
+    sys_.set_x(None)
+    rv, it, info, tr = sys_.solve(tol=1e-8, nmax=1200, trace_cap=64)
+    xo, rvo, ito, info_o, tro = oracle.bicgstab("pdfiim", A, b, F, tol=1e-8, nmax=1200, trace_cap=64)
+    m = min(20, len(tr), len(tro))
+    assert _trace_close(tr[:m], tro[:m])
+    assert info == info_o == 0 and abs(it - ito) <= max(1, int(np.ceil(0.02 * ito)))
+    sys_.set_x(None)
+    rv, outer, inner, info = sys_.solve_mixed(16, tol=1e-8, inner_tol=1e-4, nmax=1200)
+    xm = sys_.get_x()
+    sys_.close()
+    assert info == 0
+    assert np.linalg.norm(b - cmamd.csr_mv_host(A, xm)) / np.linalg.norm(b) <= 1e-8

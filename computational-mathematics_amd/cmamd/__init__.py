@@ -280,7 +280,7 @@ SOLVE_SYMBOLS = [
     "cm_system_enable_mixed", "cm_system_solve_mixed",
     "cm_system_block_set", "cm_system_block_get", "cm_system_block_solve", "cm_system_block_solve_mixed",
     "cm_system_block_graphs", "cm_system_block_release",
-    "cm_system_block_time_spmm", "cm_system_autotune", "cm_system_block_autotune", "cm_last_error", "cm_device_count",
+    "cm_system_block_time_spmm", "cm_batch_solve", "cm_system_autotune", "cm_system_block_autotune", "cm_last_error", "cm_device_count",
     "cm_server_create", "cm_server_configure", "cm_server_destroy", "cm_server_add_system", "cm_server_submit", "cm_server_wait",
     "cm_server_stats",
 ]
@@ -324,6 +324,9 @@ def _type_solve(lib):
     lib.cm_system_block_release.argtypes = [vp]
     lib.cm_system_block_time_spmm.argtypes = [vp, C.c_char, C.c_int, C.c_int, PD, PD]
     lib.cm_system_block_time_spmm.restype = C.c_int
+    lib.cm_batch_solve.argtypes = [C.c_int, C.POINTER(vp), C.POINTER(PD), C.POINTER(PD), PD, PI, C.c_int, PD,
+                                   PI, PI]
+    lib.cm_batch_solve.restype = C.c_int
     lib.cm_system_autotune.argtypes = [vp, C.c_int, PI]
     lib.cm_system_autotune.restype = C.c_int
     lib.cm_system_block_autotune.argtypes = [vp, C.c_int, C.c_int, PI]
@@ -334,7 +337,7 @@ def _type_solve(lib):
     PLL = C.POINTER(C.c_longlong)
     lib.cm_server_create.argtypes = [C.POINTER(vp), C.c_int, C.c_int, C.c_longlong]
     lib.cm_server_destroy.argtypes = [vp]
-    lib.cm_server_configure.argtypes = [vp, C.c_int, C.c_double, C.c_int]
+    lib.cm_server_configure.argtypes = [vp, C.c_int, C.c_double, C.c_int, C.c_int]
     lib.cm_server_configure.restype = C.c_int
     lib.cm_server_add_system.argtypes = [vp, vp, PI]
     lib.cm_server_submit.argtypes = [vp, C.c_int, C.c_int, PD, C.c_double, C.c_int, PLL]
@@ -620,12 +623,13 @@ class Server:
     FP16 factors.  Registered systems must outlive the server and must not be
     used directly while it runs."""
 
-    def __init__(self, slots=4, precision=32, hbm_budget=0, max_cols=16, inner_tol=1e-4, max_outer=30):
+    def __init__(self, slots=4, precision=32, hbm_budget=0, max_cols=16, inner_tol=1e-4, max_outer=30,
+                 small_n=0):
         self._lib = solve_lib()
         h = C.c_void_p()
         _check(self._lib.cm_server_create(C.byref(h), slots, precision, int(hbm_budget)))
         self._h = h
-        _check(self._lib.cm_server_configure(h, max_cols, inner_tol, max_outer))
+        _check(self._lib.cm_server_configure(h, max_cols, inner_tol, max_outer, small_n))
         self._n = {}
         self._k = {}
         self._systems = []
@@ -655,9 +659,10 @@ class Server:
                       float(tms[2]))
 
     def stats(self):
-        s = (C.c_longlong * 6)()
+        s = (C.c_longlong * 8)()
         _check(self._lib.cm_server_stats(self._h, s))
-        return dict(zip(("batches", "columns", "max_running", "evictions", "bytes", "budget"), list(s)))
+        return dict(zip(("batches", "columns", "max_running", "evictions", "bytes", "budget",
+                         "persistent_batches", "persistent_columns"), list(s)))
 
     def close(self):
         if self._h is not None:
@@ -669,3 +674,22 @@ class Server:
             self.close()
         except Exception:
             pass
+
+
+def batch_solve(items, restart_true=False):
+    """Persistent batched FP64 solves of small resident systems in one launch.
+    items: list of (System, b, x0 or None, tol, nmax).  Returns a list of
+    SolveResult (x, resval, iter, info)."""
+    lib = solve_lib()
+    m = len(items)
+    hs = (C.c_void_p * m)(*[it[0]._h for it in items])
+    bs = [np.ascontiguousarray(it[1], np.float64) for it in items]
+    xs = [np.zeros(len(b)) if it[2] is None else np.array(it[2], np.float64) for it, b in zip(items, bs)]
+    pb = (PD * m)(*[_dp(b) for b in bs])
+    px = (PD * m)(*[_dp(x) for x in xs])
+    tol = np.array([it[3] for it in items], np.float64)
+    nmax = np.array([it[4] for it in items], np.int32)
+    rv, it_, info = np.empty(m), np.empty(m, np.int32), np.empty(m, np.int32)
+    _check(lib.cm_batch_solve(m, hs, pb, px, _dp(tol), _ip(nmax), 1 if restart_true else 0, _dp(rv), _ip(it_),
+                              _ip(info)))
+    return [SolveResult(xs[i], float(rv[i]), int(it_[i]), int(info[i])) for i in range(m)]

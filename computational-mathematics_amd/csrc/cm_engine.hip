@@ -38,6 +38,7 @@
 
 #include "cm_kernels.hpp"
 #include "cm_solve.h"
+#include "cm_batch.hpp"
 #include "cm_block.hpp"
 #include "cm_trsv.hpp"
 
@@ -1671,6 +1672,120 @@ static int dropin_solve(System &S, const double *b, double *x, double tol, int N
 
 using namespace cm;
 
+// ------------------------------------------------- persistent batched solves
+
+// Host side of k_batch_solve: device copies of the system descriptors and
+// items, the per-workgroup arena and the items' b / x, grown on demand and
+// kept between calls.
+struct BatchRunner {
+    BatchSys *dsys = nullptr;
+    BatchItem *ditems = nullptr, *hitems = nullptr;  // hitems pinned
+    size_t sys_cap = 0, item_cap = 0;
+    double *arena = nullptr, *vec = nullptr;
+    size_t arena_cap = 0, vec_cap = 0;  // doubles
+    int *head = nullptr;
+    ~BatchRunner()
+    {
+        (void)hipFree(dsys);
+        (void)hipFree(ditems);
+        if (hitems) (void)hipHostFree(hitems);
+        (void)hipFree(arena);
+        (void)hipFree(vec);
+        (void)hipFree(head);
+    }
+    size_t bytes() const
+    {
+        return sys_cap * sizeof(BatchSys) + item_cap * sizeof(BatchItem) + (arena_cap + vec_cap) * sizeof(double);
+    }
+
+    template <class P>
+    static int grow(P **d, size_t &cap, size_t need)
+    {
+        if (need <= cap) return CM_OK;
+        (void)hipFree(*d);
+        *d = nullptr;
+        CM_HIP(hipMalloc((void **)d, need * sizeof(P)));
+        cap = need;
+        return CM_OK;
+    }
+
+    // Solves nitems right-hand sides: item i is column B[i] (n of system
+    // sys_of[i]) from initial guess X[i], result into X[i].
+    int run(const std::vector<const System *> &systems, int nitems, const int *sys_of, const double *const *B,
+            double *const *X, const double *tol, const int *nmax, int restart_true, double *resval, int *iter,
+            int *info, int *cycles, hipStream_t s)
+    {
+        if (nitems <= 0) return CM_OK;
+        const int ns = (int)systems.size();
+        std::vector<BatchSys> hs(ns);
+        int maxn = 0;
+        for (int q = 0; q < ns; q++) {
+            const K64 &k = *systems[q]->k64;
+            if (k.variant == 2 && k.mode != 'E') return fail(CM_ERR_UNSUPPORTED, "batched solves use apply mode E");
+            const DevCsr<double> &H = k.sym ? k.GT : k.H;
+            hs[q] = BatchSys{k.n, k.variant, max_div_steps(k.variant, k.n), 0, k.A.ptr, k.A.col, k.A.val,
+                             k.G.ptr, k.G.col, k.G.val, H.ptr, H.col, H.val, k.idv};
+            maxn = std::max(maxn, k.n);
+        }
+        // largest systems first (longest solves start early)
+        std::vector<int> order(nitems);
+        for (int i = 0; i < nitems; i++) order[i] = i;
+        std::stable_sort(order.begin(), order.end(),
+                         [&](int a, int b) { return systems[sys_of[a]]->n > systems[sys_of[b]]->n; });
+        std::vector<size_t> off(nitems);
+        size_t tot = 0;
+        for (int i = 0; i < nitems; i++) {
+            off[i] = tot;
+            tot += 2 * (size_t)systems[sys_of[i]]->n;
+        }
+        int dev = 0, cus = 256;
+        CM_HIP(hipGetDevice(&dev));
+        CM_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+        const int grid = std::min(nitems, cus);  // one PBS-thread workgroup fits a CU
+        const size_t stride = ((size_t)9 * maxn + 63) / 64 * 64;
+        CM_TRY(grow(&dsys, sys_cap, (size_t)ns));
+        CM_TRY(grow(&ditems, item_cap, (size_t)nitems));
+        if (nitems > (int)hitems_cap) {
+            if (hitems) (void)hipHostFree(hitems);
+            hitems = nullptr;
+            CM_HIP(hipHostMalloc((void **)&hitems, (size_t)nitems * sizeof(BatchItem), hipHostMallocDefault));
+            hitems_cap = nitems;
+        }
+        CM_TRY(grow(&arena, arena_cap, stride * grid));
+        CM_TRY(grow(&vec, vec_cap, tot));
+        if (!head) CM_HIP(hipMalloc((void **)&head, sizeof(int)));
+        for (int q = 0; q < nitems; q++) {
+            const int i = order[q];
+            const int n = systems[sys_of[i]]->n;
+            double *db = vec + off[i], *dx = db + n;
+            CM_HIP(hipMemcpyAsync(db, B[i], (size_t)n * sizeof(double), hipMemcpyHostToDevice, s));
+            CM_HIP(hipMemcpyAsync(dx, X[i], (size_t)n * sizeof(double), hipMemcpyHostToDevice, s));
+            hitems[q] = BatchItem{sys_of[i], nmax[i], tol[i], db, dx, 0.0, 0, 0, 0, 0};
+        }
+        CM_HIP(hipMemcpyAsync(dsys, hs.data(), ns * sizeof(BatchSys), hipMemcpyHostToDevice, s));
+        CM_HIP(hipMemcpyAsync(ditems, hitems, (size_t)nitems * sizeof(BatchItem), hipMemcpyHostToDevice, s));
+        k_zero_int<<<1, 1, 0, s>>>(head);
+        k_batch_solve<<<grid, PBS, 0, s>>>(nitems, dsys, ditems, arena, stride, head, restart_true);
+        CM_HIP(hipGetLastError());
+        CM_HIP(hipMemcpyAsync(hitems, ditems, (size_t)nitems * sizeof(BatchItem), hipMemcpyDeviceToHost, s));
+        for (int q = 0; q < nitems; q++) {
+            const int i = order[q];
+            const int n = systems[sys_of[i]]->n;
+            CM_HIP(hipMemcpyAsync(X[i], vec + off[i] + n, (size_t)n * sizeof(double), hipMemcpyDeviceToHost, s));
+        }
+        CM_HIP(hipStreamSynchronize(s));
+        for (int q = 0; q < nitems; q++) {
+            const int i = order[q];
+            if (resval) resval[i] = hitems[q].resval;
+            if (iter) iter[i] = hitems[q].iter;
+            if (info) info[i] = hitems[q].info;
+            if (cycles) cycles[i] = hitems[q].cycles;
+        }
+        return CM_OK;
+    }
+    size_t hitems_cap = 0;
+};
+
 struct cm_system_s {
     System sys;
 };
@@ -1874,6 +1989,30 @@ int cm_system_block_autotune(cm_system h, int bits, int reps, int *short_max)
     h->sys.blk->k32 = h->sys.k32.get();
     h->sys.blk->k16 = h->sys.k16.get();
     return h->sys.blk->tune(bits, reps, short_max);
+}
+
+int cm_batch_solve(int nitems, const cm_system *systems, const double *const *B, double *const *X,
+                   const double *tol, const int *NMAX, int restart_true, double *resval, int *iter, int *info)
+{
+    if (nitems < 0 || (nitems > 0 && (!systems || !B || !X || !tol || !NMAX))) return fail(CM_ERR_ARG, "null argument");
+    if (nitems == 0) return CM_OK;
+    static thread_local std::unique_ptr<BatchRunner> runner;
+    static thread_local hipStream_t stream = nullptr;
+    if (!stream) CM_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+    if (!runner) runner = std::make_unique<BatchRunner>();
+    // distinct systems, in first-appearance order
+    std::vector<const System *> sys;
+    std::vector<int> sys_of(nitems);
+    for (int i = 0; i < nitems; i++) {
+        if (!systems[i] || !B[i] || !X[i]) return fail(CM_ERR_ARG, "null system or vector");
+        const System *p = &systems[i]->sys;
+        int q = 0;
+        while (q < (int)sys.size() && sys[q] != p) q++;
+        if (q == (int)sys.size()) sys.push_back(p);
+        sys_of[i] = q;
+    }
+    return runner->run(sys, nitems, sys_of.data(), B, X, tol, NMAX, restart_true, resval, iter, info, nullptr,
+                       stream);
 }
 
 int cm_system_block_release(cm_system h)

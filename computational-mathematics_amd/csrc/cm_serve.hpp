@@ -19,6 +19,10 @@
 //    batches to finish, and a request that can never fit is failed.
 //  * precision: 64 = FP64 block BiCGSTAB; 32 / 16 = FP64 refinement around
 //    the FP32 inner block solve (FP32 or FP16-with-row-scale factors).
+//  * small systems (n <= small_n, when set): every pending request of any
+//    small system goes into one launch of the persistent batched kernel, one
+//    workgroup per right-hand side (FP64; under refinement precision the
+//    items restart until their true residual is below tol).
 // Results do not depend on which other requests share a batch beyond the
 // summation order of the column reductions (block width): every column runs
 // its own recurrence and exit tests.
@@ -68,6 +72,12 @@ static size_t workspace_bytes(const System &S)
 
 struct Server {
     int device = 0, precision = 32, max_cols = 16, max_outer = 30;
+    int small_n = 0;         // systems with n <= small_n go to the persistent batched kernel (0: never)
+    int max_items = 4096;    // right-hand sides per persistent launch
+    int cus = 256;
+    std::vector<std::unique_ptr<BatchRunner>> runners;  // one per worker
+    std::vector<size_t> runner_bytes;
+    long long pbatches = 0, pcolumns = 0;
     double inner_tol = 1e-4;
     size_t budget = 0;
     size_t reserved = 0;  // workspace bytes reserved by running batches, not yet created
@@ -89,6 +99,7 @@ struct Server {
     {
         size_t b = reserved;
         for (const auto &s : systems) b += s.resident + s.ws;
+        for (size_t q : runner_bytes) b += q;
         return b;
     }
 
@@ -125,12 +136,78 @@ struct Server {
         }
     }
 
-    // Picks the next batch (caller holds mu).  Returns 1 with `batch` and
-    // `si` set, 0 when nothing can start now.
-    int pick(std::vector<Request *> &batch, int &si, size_t &res)
+    // Device bytes worker w's batch runner would add for these requests.
+    size_t persistent_need(int w, const std::vector<Request *> &reqs_) const
+    {
+        size_t cols = 0, vec = 0;
+        int maxn = 0;
+        for (const Request *r : reqs_) {
+            const int n = systems[r->sys].sys->n;
+            cols += r->k;
+            vec += 2 * (size_t)n * r->k;
+            maxn = std::max(maxn, n);
+        }
+        const size_t arena = std::min(cols, (size_t)cus) * ((9 * (size_t)maxn + 63) / 64 * 64);
+        const size_t want = (arena + vec) * sizeof(double) + cols * sizeof(BatchItem) + systems.size() * sizeof(BatchSys);
+        return want > runner_bytes[w] ? want - runner_bytes[w] : 0;
+    }
+
+    bool is_small(const Request *r) const { return small_n > 0 && systems[r->sys].sys->n <= small_n; }
+
+    // Admission of `need` more bytes (caller holds mu): evicts idle
+    // workspaces when that makes room.  1: admitted; 0: wait; -1: never fits.
+    int admit(size_t need, int keep)
+    {
+        if (accounted() + need > budget) evict_for(need, keep);
+        if (accounted() + need <= budget) return 1;
+        return running > 0 ? 0 : -1;
+    }
+
+    void reject(size_t i)
+    {
+        Request *r = pending[i];
+        r->rc = CM_ERR_ARG;
+        r->err = "request does not fit the HBM budget";
+        r->done = true;
+        r->t_done = Clock::now();
+        pending.erase(pending.begin() + i);
+        cv_done.notify_all();
+    }
+
+    // Picks the next batch for worker w (caller holds mu).  Returns 1 for a
+    // block batch of system si, 2 for a persistent batch of small systems,
+    // 0 when nothing can start now.
+    int pick(int w, std::vector<Request *> &batch, int &si, size_t &res)
     {
         for (size_t i = 0; i < pending.size(); i++) {
             Request *r = pending[i];
+            if (is_small(r)) {
+                // every pending small-system request, oldest first
+                std::vector<size_t> take;
+                std::vector<Request *> sel;
+                size_t cols = 0;
+                for (size_t q = i; q < pending.size(); q++) {
+                    Request *c = pending[q];
+                    if (!is_small(c) || (q != i && cols + c->k > (size_t)max_items)) continue;
+                    take.push_back(q);
+                    sel.push_back(c);
+                    cols += c->k;
+                }
+                const size_t need = persistent_need(w, sel);
+                const int a = admit(need, -1);
+                if (a == 0) return 0;
+                if (a < 0) {
+                    reject(i);
+                    i--;
+                    continue;
+                }
+                reserved += need;
+                res = need;
+                for (auto it = take.rbegin(); it != take.rend(); ++it) pending.erase(pending.begin() + *it);
+                batch = sel;
+                si = -1;
+                return 2;
+            }
             if (systems[r->sys].busy) continue;
             const int s = r->sys;
             std::vector<size_t> take;
@@ -144,16 +221,10 @@ struct Server {
             }
             const int kb = block_width(cols);
             size_t need = workspace_need(s, kb);
-            if (accounted() + need > budget) evict_for(need, s);
-            if (accounted() + need > budget) {
-                if (running > 0) return 0;  // wait for memory to free up
-                // nothing running and still too big: this request never fits
-                r->rc = CM_ERR_ARG;
-                r->err = "request does not fit the HBM budget";
-                r->done = true;
-                r->t_done = Clock::now();
-                pending.erase(pending.begin() + i);
-                cv_done.notify_all();
+            const int a = admit(need, s);
+            if (a == 0) return 0;  // wait for memory to free up
+            if (a < 0) {           // nothing running and still too big: never fits
+                reject(i);
                 i--;
                 continue;
             }
@@ -222,35 +293,91 @@ struct Server {
         return CM_OK;
     }
 
+    // Small systems: every column of the batch is one item of the persistent
+    // kernel (FP64; with refinement precision the items restart until their
+    // true residual is below tol).
+    int run_persistent(int w, std::vector<Request *> &batch)
+    {
+        std::vector<const System *> sl;
+        std::vector<int> local(systems.size(), -1);
+        std::vector<int> sys_of;
+        std::vector<const double *> B;
+        std::vector<double *> X;
+        std::vector<double> tol;
+        std::vector<int> nmax;
+        for (Request *r : batch) {
+            if (local[r->sys] < 0) {
+                local[r->sys] = (int)sl.size();
+                sl.push_back(systems[r->sys].sys);
+            }
+            const int n = systems[r->sys].sys->n;
+            r->X.assign((size_t)n * r->k, 0.0);
+            for (int j = 0; j < r->k; j++) {
+                sys_of.push_back(local[r->sys]);
+                B.push_back(r->B.data() + (size_t)j * n);
+                X.push_back(r->X.data() + (size_t)j * n);
+                tol.push_back(r->tol);
+                nmax.push_back(r->nmax);
+            }
+        }
+        const int m = (int)B.size();
+        std::vector<double> rv(m);
+        std::vector<int> it(m), info(m), cyc(m);
+        const auto t0 = Clock::now();
+        for (Request *r : batch) r->t_start = t0;
+        CM_TRY(runners[w]->run(sl, m, sys_of.data(), B.data(), X.data(), tol.data(), nmax.data(),
+                               precision != 64 ? 1 : 0, rv.data(), it.data(), info.data(), cyc.data(), streams[w]));
+        const auto t1 = Clock::now();
+        int c = 0;
+        for (Request *r : batch) {
+            r->resval.assign(rv.begin() + c, rv.begin() + c + r->k);
+            r->info.assign(info.begin() + c, info.begin() + c + r->k);
+            r->outer = *std::max_element(cyc.begin() + c, cyc.begin() + c + r->k);
+            r->inner = *std::max_element(it.begin() + c, it.begin() + c + r->k);
+            r->batch_cols = m;
+            r->t_done = t1;
+            c += r->k;
+        }
+        return CM_OK;
+    }
+
     void worker(int w)
     {
         (void)hipSetDevice(device);
         for (;;) {
             std::vector<Request *> batch;
-            int si = -1;
+            int si = -1, kind = 0;
             size_t res = 0;
             {
                 std::unique_lock<std::mutex> lk(mu);
                 for (;;) {
                     if (stop) return;
-                    if (pick(batch, si, res)) break;
+                    if ((kind = pick(w, batch, si, res)) != 0) break;
                     cv_work.wait(lk);
                 }
-                systems[si].busy = true;
-                systems[si].last_use = ++tick;
+                if (kind == 1) {
+                    systems[si].busy = true;
+                    systems[si].last_use = ++tick;
+                }
                 running++;
                 max_running = std::max(max_running, running);
             }
-            const int rc = run_batch(w, si, batch);
+            const int rc = kind == 1 ? run_batch(w, si, batch) : run_persistent(w, batch);
             const std::string err = rc == CM_OK ? std::string() : std::string(cm_last_error());
             {
                 std::lock_guard<std::mutex> lk(mu);
                 // the reservation becomes the workspace's own size
                 reserved -= std::min(reserved, res);
-                systems[si].ws = workspace_bytes(*systems[si].sys);
-                systems[si].busy = false;
+                if (kind == 1) {
+                    systems[si].ws = workspace_bytes(*systems[si].sys);
+                    systems[si].busy = false;
+                    batches++;
+                } else {
+                    runner_bytes[w] = runners[w]->bytes();
+                    pbatches++;
+                    for (Request *r : batch) pcolumns += r->k;
+                }
                 running--;
-                batches++;
                 for (Request *r : batch) {
                     columns += r->k;
                     if (rc != CM_OK) {
@@ -294,27 +421,31 @@ int cm_server_create(cm_server *out, int slots, int precision, long long hbm_bud
         CM_HIP(hipMemGetInfo(&fr, &tot));
         S.budget = tot / 10 * 9;
     }
+    CM_HIP(hipDeviceGetAttribute(&S.cus, hipDeviceAttributeMultiprocessorCount, S.device));
     for (int w = 0; w < slots; w++) {
         hipStream_t s;
         CM_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
         S.streams.push_back(s);
+        S.runners.push_back(std::make_unique<BatchRunner>());
+        S.runner_bytes.push_back(0);
     }
     for (int w = 0; w < slots; w++) S.workers.emplace_back([&S, w] { S.worker(w); });
     *out = h.release();
     return CM_OK;
 }
 
-int cm_server_configure(cm_server h, int max_cols, double inner_tol, int max_outer)
+int cm_server_configure(cm_server h, int max_cols, double inner_tol, int max_outer, int small_n)
 {
     using namespace cm;
     if (!h) return fail(CM_ERR_ARG, "null handle");
     if (max_cols < 1 || max_cols > 16) return fail(CM_ERR_ARG, "max_cols must be 1..16");
-    if (!(inner_tol > 0.0) || max_outer < 0) return fail(CM_ERR_ARG, "bad refinement settings");
+    if (!(inner_tol > 0.0) || max_outer < 0 || small_n < 0) return fail(CM_ERR_ARG, "bad settings");
     Server &S = h->srv;
     std::lock_guard<std::mutex> lk(S.mu);
     S.max_cols = max_cols;
     S.inner_tol = inner_tol;
     S.max_outer = max_outer;
+    S.small_n = small_n;
     return CM_OK;
 }
 
@@ -430,8 +561,9 @@ int cm_server_wait(cm_server h, long long req_id, double *X, double *resval, int
     return CM_OK;
 }
 
-// stats: batches, columns solved, most batches running at once, workspace
-// evictions, bytes accounted, budget.
+// stats: block batches, columns solved, most batches running at once,
+// workspace evictions, bytes accounted, budget, persistent batches, their
+// columns.
 int cm_server_stats(cm_server h, long long *stats)
 {
     using namespace cm;
@@ -444,6 +576,8 @@ int cm_server_stats(cm_server h, long long *stats)
     stats[3] = S.evictions;
     stats[4] = (long long)S.accounted();
     stats[5] = (long long)S.budget;
+    stats[6] = S.pbatches;
+    stats[7] = S.pcolumns;
     return CM_OK;
 }
 

@@ -34,10 +34,13 @@ typedef struct cm_server_s *cm_server;
 int cm_server_create(cm_server *srv, int slots, int precision, long long hbm_budget);
 int cm_server_destroy(cm_server srv);
 /* Batching and refinement settings (before the first submit): at most
- * max_cols (1..16) right-hand sides per batch -- 1 disables merging; a wider
- * request still runs, alone -- and the inner tolerance / refinement steps of
- * precision 32/16. */
-int cm_server_configure(cm_server srv, int max_cols, double inner_tol, int max_outer);
+ * max_cols (1..16) right-hand sides per block batch -- 1 disables merging; a
+ * wider request still runs, alone -- the inner tolerance / refinement steps
+ * of precision 32/16, and small_n: requests on systems of at most small_n
+ * rows are solved together, every pending one of any small system in one
+ * launch of the persistent batched kernel (cm_batch_solve; FP64, with the
+ * true-residual restart under precision 32/16); 0 disables it. */
+int cm_server_configure(cm_server srv, int max_cols, double inner_tol, int max_outer, int small_n);
 /* Registers a system (apply mode E for ACIFIM); enables its mixed-precision
  * copies for precision 32/16.  sys_id: the id requests name. */
 int cm_server_add_system(cm_server srv, cm_system sys, int *sys_id);
@@ -50,8 +53,9 @@ int cm_server_submit(cm_server srv, int sys_id, int k, const double *B, double t
  * times_ms = {queue, service, latency} (arrival = submit). */
 int cm_server_wait(cm_server srv, long long req_id, double *X, double *resval, int *info, int *counts,
                    double *times_ms);
-/* stats = {batches, columns, most batches running at once, workspace
- * evictions, bytes accounted, budget}. */
+/* stats[8] = {block batches, columns, most batches running at once,
+ * workspace evictions, bytes accounted, budget, persistent batches, their
+ * columns}. */
 int cm_server_stats(cm_server srv, long long *stats);
 
 #ifdef __cplusplus

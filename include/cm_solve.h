@@ -124,6 +124,21 @@ int cm_system_block_get(cm_system sys, double *X);
 /* Block passes are replayed from hipGraphs captured on first use (on, the
  * default) or launched kernel by kernel (off); the results are identical. */
 int cm_system_block_graphs(cm_system sys, int on);
+/* Persistent batched solves: nitems right-hand sides of small resident
+ * systems (item i: system systems[i], column B[i], initial guess and result
+ * X[i], tol[i], NMAX[i]; systems may repeat and differ in size) in one
+ * launch.  Each workgroup of a persistent grid pulls items from a device
+ * queue and runs the whole FP64 reference loop (AFIIM/src/bicgstab.cpp:
+ * 149-281) for one item with workgroup barriers; largest systems go first.
+ * restart_true = 1 restarts an item that converged on the recursive residual
+ * while its true residual is above tol (at most 4 cycles, NMAX passes in
+ * total); 0 is the reference loop.  Apply mode E; per-item resval, iter,
+ * info as in cm_system_solve.  Suited to systems whose solve is launch-bound
+ * (up to some 10^4-10^5 rows); work vectors take 9 n doubles per resident
+ * workgroup. */
+int cm_batch_solve(int nitems, const cm_system *systems, const double *const *B, double *const *X,
+                   const double *tol, const int *NMAX, int restart_true, double *resval, int *iter, int *info);
+
 /* Kernel timing of the current block's SpMM y = M X, M = 'A', 'G' or 'H'
  * (first factor; G^T on the symmetric PDFIIM branch), in FP64 (bits 64) or
  * the inner precision (32, 16; cm_system_enable_mixed first): reps

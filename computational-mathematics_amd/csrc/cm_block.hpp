@@ -148,7 +148,63 @@ __device__ __forceinline__ void block_col_partials(double (&acc)[K > 0 ? K : 1],
         }
 }
 
-// Short rows: KB lanes per row, rows from the list rows[0..cnt).
+// Columns per lane in the SpMM kernels: a lane loads 16 bytes of an x row
+// (4 FP32 or 2 FP64 columns) per nonzero, or the whole row when it is
+// narrower; lanes per row = KB / CPL.
+template <class T, int KB>
+struct ColSplit {
+    static constexpr int CPL = (int)(16 / sizeof(T)) < KB ? (int)(16 / sizeof(T)) : KB;
+    static constexpr int LPRW = KB / CPL;
+};
+
+// CPL consecutive T from p (aligned to CPL * sizeof(T)).
+template <class T, int CPL>
+__device__ __forceinline__ void ld_cols(const T *p, T (&v)[CPL])
+{
+    typedef T vec_t __attribute__((ext_vector_type(CPL)));
+    if constexpr (CPL == 1) {
+        v[0] = *p;
+    } else {
+        const vec_t q = *reinterpret_cast<const vec_t *>(p);
+#pragma unroll
+        for (int i = 0; i < CPL; i++) v[i] = q[i];
+    }
+}
+
+// Workgroup sums of per-lane partials acc[i][k] (column lane_group * CPL + i,
+// value k): xor shuffles over the lanes that share a column group, LDS over
+// waves, partials[(slot * K + k) * KB + column].  Fixed order.
+template <int K, int KB, int CPL>
+__device__ __forceinline__ void block_colgroup_partials(double (&acc)[CPL][K > 0 ? K : 1], double *lds,
+                                                        double *partials, int slot)
+{
+    if constexpr (K == 0) return;
+    constexpr int LPRW = KB / CPL;
+    const int lane = threadIdx.x & (WAVE - 1), w = threadIdx.x / WAVE;
+#pragma unroll
+    for (int i = 0; i < CPL; i++)
+#pragma unroll
+        for (int k = 0; k < K; k++)
+            for (int off = WAVE / 2; off >= LPRW; off >>= 1) acc[i][k] += __shfl_xor(acc[i][k], off, WAVE);
+    __syncthreads();
+    if (lane < LPRW)
+#pragma unroll
+        for (int i = 0; i < CPL; i++)
+#pragma unroll
+            for (int k = 0; k < K; k++) lds[(w * K + k) * KB + lane * CPL + i] = acc[i][k];
+    __syncthreads();
+    if (threadIdx.x < KB)
+#pragma unroll
+        for (int k = 0; k < K; k++) {
+            double s = 0.0;
+            for (int q = 0; q < BS / WAVE; q++) s += lds[(q * K + k) * KB + threadIdx.x];
+            partials[((size_t)slot * K + k) * KB + threadIdx.x] = s;
+        }
+}
+
+// Short rows, rows from the list rows[0..cnt): LPRW lanes per row, each
+// with CPL columns; the row's nonzeros in order, four index / value loads
+// issued ahead of their x loads.
 template <class V, class T, class Acc, int KB, class Epi>
 __global__ __launch_bounds__(BS) void spmm_short(int cnt, const int *__restrict__ rows, const int *__restrict__ ptr,
                                                  const int *__restrict__ col, const V *__restrict__ val,
@@ -156,24 +212,55 @@ __global__ __launch_bounds__(BS) void spmm_short(int cnt, const int *__restrict_
                                                  Epi epi, const BlockFlags *fl, double *partials, int slot0)
 {
     if (fl && fl->all_done) return;
-    __shared__ double red[(BS / WAVE) * (Epi::K > 0 ? Epi::K : 1) * KB];
-    double acc[Epi::K > 0 ? Epi::K : 1];
+    constexpr int CPL = ColSplit<T, KB>::CPL, LPRW = ColSplit<T, KB>::LPRW, KK = Epi::K > 0 ? Epi::K : 1;
+    __shared__ double red[(BS / WAVE) * KK * KB];
+    double acc[CPL][KK];
 #pragma unroll
-    for (int k = 0; k < (Epi::K > 0 ? Epi::K : 1); k++) acc[k] = 0.0;
-    const int j = threadIdx.x & (KB - 1);
-    const int q = xcd_block(blockIdx.x, gridDim.x) * (BS / KB) + threadIdx.x / KB;
+    for (int i = 0; i < CPL; i++)
+#pragma unroll
+        for (int k = 0; k < KK; k++) acc[i][k] = 0.0;
+    const int c0 = (threadIdx.x % LPRW) * CPL;
+    const int q = xcd_block(blockIdx.x, gridDim.x) * (BS / LPRW) + threadIdx.x / LPRW;
     if (q < cnt) {
         const int r = rows[q];
-        Acc s = 0;
-        for (int k = ptr[r]; k < ptr[r + 1]; k++)
-            s += load_val<V, Acc>(val, k, false) * (Acc)x[(size_t)col[k] * KB + j];
-        if (rowscale) s *= (Acc)rowscale[r];
-        epi.row((size_t)r * KB + j, j, s, acc);
+        Acc s[CPL];
+#pragma unroll
+        for (int i = 0; i < CPL; i++) s[i] = 0;
+        int k = ptr[r];
+        const int e = ptr[r + 1];
+        for (; k + 4 <= e; k += 4) {
+            int cc[4];
+            Acc vv[4];
+#pragma unroll
+            for (int u = 0; u < 4; u++) {
+                cc[u] = col[k + u];
+                vv[u] = load_val<V, Acc>(val, k + u, false);
+            }
+#pragma unroll
+            for (int u = 0; u < 4; u++) {
+                T xv[CPL];
+                ld_cols<T, CPL>(x + (size_t)cc[u] * KB + c0, xv);
+#pragma unroll
+                for (int i = 0; i < CPL; i++) s[i] += vv[u] * (Acc)xv[i];
+            }
+        }
+        for (; k < e; k++) {
+            T xv[CPL];
+            ld_cols<T, CPL>(x + (size_t)col[k] * KB + c0, xv);
+            const Acc vk = load_val<V, Acc>(val, k, false);
+#pragma unroll
+            for (int i = 0; i < CPL; i++) s[i] += vk * (Acc)xv[i];
+        }
+        const Acc sc = rowscale ? (Acc)rowscale[r] : (Acc)1;
+#pragma unroll
+        for (int i = 0; i < CPL; i++) epi.row((size_t)r * KB + c0 + i, c0 + i, rowscale ? s[i] * sc : s[i], acc[i]);
     }
-    block_col_partials<Epi::K, KB>(acc, red, partials, slot0 + blockIdx.x);
+    block_colgroup_partials<Epi::K, KB, CPL>(acc, red, partials, slot0 + blockIdx.x);
 }
 
-// Long rows: one wave per row; lane = group g (64/KB of them) x column j.
+// Long rows: one wave per row; lane = column group (LPRW of them) + LPRW x
+// nonzero group (64 / LPRW groups striding over the row), groups summed with
+// xor shuffles.
 template <class V, class T, class Acc, int KB, class Epi>
 __global__ __launch_bounds__(BS) void spmm_long(int cnt, const int *__restrict__ rows, const int *__restrict__ ptr,
                                                 const int *__restrict__ col, const V *__restrict__ val,
@@ -181,51 +268,101 @@ __global__ __launch_bounds__(BS) void spmm_long(int cnt, const int *__restrict__
                                                 const BlockFlags *fl, double *partials, int slot0)
 {
     if (fl && fl->all_done) return;
-    __shared__ double red[(BS / WAVE) * (Epi::K > 0 ? Epi::K : 1) * KB];
-    double acc[Epi::K > 0 ? Epi::K : 1];
+    constexpr int CPL = ColSplit<T, KB>::CPL, LPRW = ColSplit<T, KB>::LPRW, KK = Epi::K > 0 ? Epi::K : 1;
+    constexpr int G = WAVE / LPRW;
+    __shared__ double red[(BS / WAVE) * KK * KB];
+    double acc[CPL][KK];
 #pragma unroll
-    for (int k = 0; k < (Epi::K > 0 ? Epi::K : 1); k++) acc[k] = 0.0;
-    constexpr int G = WAVE / KB;
+    for (int i = 0; i < CPL; i++)
+#pragma unroll
+        for (int k = 0; k < KK; k++) acc[i][k] = 0.0;
     const int lane = threadIdx.x & (WAVE - 1);
-    const int j = lane & (KB - 1), g = lane / KB;
+    const int c0 = (lane % LPRW) * CPL, g = lane / LPRW;
     const int q = blockIdx.x * (BS / WAVE) + threadIdx.x / WAVE;
-    Acc s = 0;
+    Acc s[CPL];
+#pragma unroll
+    for (int i = 0; i < CPL; i++) s[i] = 0;
     int r = -1;
     if (q < cnt) {
         r = rows[q];
-        for (int k = ptr[r] + g; k < ptr[r + 1]; k += G)
-            s += load_val<V, Acc>(val, k, false) * (Acc)x[(size_t)col[k] * KB + j];
+        const int e = ptr[r + 1];
+        int k = ptr[r] + g;
+        for (; k + 3 * G < e; k += 4 * G) {
+            int cc[4];
+            Acc vv[4];
+#pragma unroll
+            for (int u = 0; u < 4; u++) {
+                cc[u] = col[k + u * G];
+                vv[u] = load_val<V, Acc>(val, k + u * G, false);
+            }
+#pragma unroll
+            for (int u = 0; u < 4; u++) {
+                T xv[CPL];
+                ld_cols<T, CPL>(x + (size_t)cc[u] * KB + c0, xv);
+#pragma unroll
+                for (int i = 0; i < CPL; i++) s[i] += vv[u] * (Acc)xv[i];
+            }
+        }
+        for (; k < e; k += G) {
+            T xv[CPL];
+            ld_cols<T, CPL>(x + (size_t)col[k] * KB + c0, xv);
+            const Acc vk = load_val<V, Acc>(val, k, false);
+#pragma unroll
+            for (int i = 0; i < CPL; i++) s[i] += vk * (Acc)xv[i];
+        }
     }
-    for (int off = WAVE / 2; off >= KB; off >>= 1) s += __shfl_xor(s, off, WAVE);
+#pragma unroll
+    for (int i = 0; i < CPL; i++)
+        for (int off = WAVE / 2; off >= LPRW; off >>= 1) s[i] += __shfl_xor(s[i], off, WAVE);
     if (q < cnt && g == 0) {
-        if (rowscale) s *= (Acc)rowscale[r];
-        epi.row((size_t)r * KB + j, j, s, acc);
+        const Acc sc = rowscale ? (Acc)rowscale[r] : (Acc)1;
+#pragma unroll
+        for (int i = 0; i < CPL; i++) epi.row((size_t)r * KB + c0 + i, c0 + i, rowscale ? s[i] * sc : s[i], acc[i]);
     }
-    block_col_partials<Epi::K, KB>(acc, red, partials, slot0 + blockIdx.x);
+    block_colgroup_partials<Epi::K, KB, CPL>(acc, red, partials, slot0 + blockIdx.x);
 }
 
-// One workgroup: column j's nblk partials of K values in a fixed order, then
-// fin(state[j]) for every live column not done.  Sets all_done.
+// One workgroup sums the nblk partials of every column at once: thread t
+// takes column t % KB over the slots t / KB, t / KB + FBS / KB, ... (so a
+// wave reads whole KB-column rows of the partials, coalesced), the lanes of a
+// column are combined with xor shuffles, the waves in order through LDS, and
+// thread j runs fin(state[j]) for live column j.  Fixed order throughout.
+// Sets all_done when every live column is done.
 template <int K, int KB, class Fin>
 __global__ __launch_bounds__(FBS) void k_block_finalize(int nblk, const double *__restrict__ partials,
                                                         SolveState *st, BlockFlags *fl, Fin fin, int update_all)
 {
     if (fl->all_done) return;
-    __shared__ double red[K * FBS / WAVE];
-    for (int j = 0; j < fl->ncols; j++) {
-        double s[K];
+    __shared__ double red[(FBS / WAVE) * K * KB];
+    const int j = threadIdx.x % KB, lane = threadIdx.x & (WAVE - 1), w = threadIdx.x / WAVE;
+    double s[K];
 #pragma unroll
-        for (int k = 0; k < K; k++) s[k] = 0.0;
-        for (int b = threadIdx.x; b < nblk; b += FBS)
+    for (int k = 0; k < K; k++) s[k] = 0.0;
+    for (int b = threadIdx.x / KB; b < nblk; b += FBS / KB)
 #pragma unroll
-            for (int k = 0; k < K; k++) s[k] += partials[((size_t)b * K + k) * KB + j];
-        block_sum<K, FBS>(s, red);
-        if (threadIdx.x == 0 && !st[j].done) fin.fin(&st[j], s);
-        __syncthreads();
+        for (int k = 0; k < K; k++) s[k] += partials[((size_t)b * K + k) * KB + j];
+#pragma unroll
+    for (int k = 0; k < K; k++)
+        for (int off = WAVE / 2; off >= KB; off >>= 1) s[k] += __shfl_xor(s[k], off, WAVE);
+    if (lane < KB)
+#pragma unroll
+        for (int k = 0; k < K; k++) red[(w * K + k) * KB + lane] = s[k];
+    __syncthreads();
+    const int ncols = fl->ncols;
+    if (threadIdx.x < ncols) {
+        double tot[K];
+#pragma unroll
+        for (int k = 0; k < K; k++) {
+            double q = 0.0;
+            for (int v = 0; v < FBS / WAVE; v++) q += red[(v * K + k) * KB + threadIdx.x];
+            tot[k] = q;
+        }
+        if (!st[threadIdx.x].done) fin.fin(&st[threadIdx.x], tot);
     }
+    __syncthreads();
     if (threadIdx.x == 0 && update_all) {
         int all = 1;
-        for (int j = 0; j < fl->ncols; j++) all &= (st[j].done != 0);
+        for (int c = 0; c < ncols; c++) all &= (st[c].done != 0);
         fl->all_done = all;
     }
 }

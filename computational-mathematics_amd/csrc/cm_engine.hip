@@ -143,6 +143,17 @@ static int dev_upload(T **d, const T *h, size_t count, hipStream_t s)
     return CM_OK;
 }
 
+// Matrix arrays: nnz values followed by CSR_PAD zeros (the 16-byte loads of
+// spmv_vl may read past the last nonzero).
+template <class T>
+static int dev_upload_padded(T **d, const T *h, size_t count, hipStream_t s)
+{
+    CM_TRY(dev_alloc(d, count + CSR_PAD));
+    if (h && count) CM_HIP(hipMemcpyAsync(*d, h, count * sizeof(T), hipMemcpyHostToDevice, s));
+    CM_HIP(hipMemsetAsync(*d + count, 0, CSR_PAD * sizeof(T), s));
+    return CM_OK;
+}
+
 static inline int vec_grid(int n)
 {
     int g = (n + BS - 1) / BS;
@@ -221,8 +232,8 @@ static int upload_csr(DevCsr<double> &m, int n, const int *ptr, const int *col, 
     std::vector<int> blk = make_row_blocks(n, ptr);
     m.nblk = (int)blk.size() - 1;
     CM_TRY(dev_upload(&m.ptr, ptr, (size_t)n + 1, s));
-    CM_TRY(dev_upload(&m.col, col, (size_t)m.nnz, s));
-    CM_TRY(dev_upload(&m.val, val, (size_t)m.nnz, s));
+    CM_TRY(dev_upload_padded(&m.col, col, (size_t)m.nnz, s));
+    CM_TRY(dev_upload_padded(&m.val, val, (size_t)m.nnz, s));
     CM_TRY(dev_upload(&m.blk, blk.data(), blk.size(), s));
     CM_HIP(hipStreamSynchronize(s));  // blk is a host temporary
     return CM_OK;
@@ -245,7 +256,8 @@ static int cast_csr(DevCsr<V> &dst, const DevCsr<double> &src, hipStream_t s)
     dst.nlev = src.nlev;
     dst.lrows = src.lrows;
     dst.loff = src.loff;
-    CM_TRY(dev_alloc(&dst.val, (size_t)dst.nnz));
+    CM_TRY(dev_alloc(&dst.val, (size_t)dst.nnz + CSR_PAD));
+    CM_HIP(hipMemsetAsync(dst.val + dst.nnz, 0, CSR_PAD * sizeof(V), s));
     if constexpr (std::is_same<V, __half>::value) {
         CM_TRY(dev_alloc(&dst.rowscale, (size_t)dst.n));
         k_half_rows<<<(dst.n + BS - 1) / BS, BS, 0, s>>>(dst.n, src.ptr, src.val, dst.val, dst.rowscale);
@@ -555,6 +567,19 @@ struct Krylov {
         SolveState *kst = fst ? fst : st;
         SolveState *gate = use_state ? kst : nullptr;
         timed(cat, spmv_bytes<V, Epi>(m), [&] {
+            if constexpr (std::is_same<V, __half>::value) {
+                // 2-byte values: the vector-load variant (fewer load instructions)
+                if (m.tile == 1024)
+                    spmv_vl<1024, V, T, Acc, Epi><<<m.nblk, BS, 0, stream>>>(m.ptr, m.col, m.val, m.rowscale,
+                                                                            m.blk, xin, epi, gate, partials);
+                else if (m.tile == 4096)
+                    spmv_vl<4096, V, T, Acc, Epi><<<m.nblk, BS, 0, stream>>>(m.ptr, m.col, m.val, m.rowscale,
+                                                                            m.blk, xin, epi, gate, partials);
+                else
+                    spmv_vl<2048, V, T, Acc, Epi><<<m.nblk, BS, 0, stream>>>(m.ptr, m.col, m.val, m.rowscale,
+                                                                            m.blk, xin, epi, gate, partials);
+                return;
+            }
             if (m.tile == 1024)
                 spmv_kernel<1024, V, T, Acc, Epi><<<m.nblk, BS, 0, stream>>>(m.ptr, m.col, m.val, m.rowscale,
                                                                             m.blk, xin, epi, gate, partials);
@@ -920,7 +945,8 @@ struct BlockKrylov {
 
     size_t ne() const { return (size_t)n * KB; }
     int vgrid() const { return (int)std::max<size_t>(1, std::min<size_t>((ne() + BS - 1) / BS, MAX_RED_BLOCKS)); }
-    static int short_grid(const RowBins &bn) { return (bn.ns + (BS / KB) - 1) / (BS / KB); }
+    static constexpr int SROWS = BS / ColSplit<T, KB>::LPRW;  // short rows per workgroup
+    static int short_grid(const RowBins &bn) { return (bn.ns + SROWS - 1) / SROWS; }
     static int long_grid(const RowBins &bn) { return (bn.nl + (BS / WAVE) - 1) / (BS / WAVE); }
 
     // device bytes of a block solver of this type for an n-row system whose

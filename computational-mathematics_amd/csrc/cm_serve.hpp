@@ -77,6 +77,10 @@ struct Server {
     int cus = 256;
     std::vector<std::unique_ptr<BatchRunner>> runners;  // one per worker
     std::vector<size_t> runner_bytes;
+    // per-worker pinned staging for a batch's right-hand sides and solutions
+    // (true asynchronous copies), grown on demand
+    std::vector<double *> pin;
+    std::vector<size_t> pin_cap;
     long long pbatches = 0, pcolumns = 0;
     double inner_tol = 1e-4;
     size_t budget = 0;
@@ -240,17 +244,30 @@ struct Server {
         return 0;
     }
 
+    int pinned(int w, size_t count)
+    {
+        if (count <= pin_cap[w]) return CM_OK;
+        if (pin[w]) (void)hipHostFree(pin[w]);
+        pin[w] = nullptr;
+        pin_cap[w] = 0;
+        CM_HIP(hipHostMalloc((void **)&pin[w], count * sizeof(double), hipHostMallocDefault));
+        pin_cap[w] = count;
+        return CM_OK;
+    }
+
     int run_batch(int w, int si, std::vector<Request *> &batch)
     {
         System &S = *systems[si].sys;
         const int n = S.n;
         int k = 0;
         for (Request *r : batch) k += r->k;
-        std::vector<double> B((size_t)n * k), X((size_t)n * k), rv(k);
+        std::vector<double> rv(k);
         std::vector<int> it(k, 0), info(k, 0);
+        CM_TRY(pinned(w, (size_t)n * k));
+        double *B = pin[w], *X = pin[w];  // the solution overwrites B once it is on the device
         size_t off = 0;
         for (Request *r : batch) {
-            std::memcpy(&B[off], r->B.data(), r->B.size() * sizeof(double));
+            std::memcpy(B + off, r->B.data(), r->B.size() * sizeof(double));
             off += r->B.size();
         }
         const auto t0 = Clock::now();
@@ -262,7 +279,7 @@ struct Server {
         blk.set_graphs(S.block_graphs);
         blk.set_stream(streams[w]);
         int outer = 0, inner = 0;
-        int rc = blk.set(k, B.data(), nullptr);
+        int rc = blk.set(k, B, nullptr);
         const Request &r0 = *batch[0];
         if (rc == CM_OK) {
             if (precision == 64) {
@@ -273,14 +290,14 @@ struct Server {
                                      info.data());
             }
         }
-        if (rc == CM_OK) rc = blk.get(X.data());
+        if (rc == CM_OK) rc = blk.get(X);
         blk.set_stream(S.stream);
         if (rc != CM_OK) return rc;
         const auto t1 = Clock::now();
         off = 0;
         int c = 0;
         for (Request *r : batch) {
-            r->X.assign(X.begin() + off, X.begin() + off + r->B.size());
+            r->X.assign(X + off, X + off + r->B.size());
             r->resval.assign(rv.begin() + c, rv.begin() + c + r->k);
             r->info.assign(info.begin() + c, info.begin() + c + r->k);
             r->outer = outer;
@@ -428,6 +445,8 @@ int cm_server_create(cm_server *out, int slots, int precision, long long hbm_bud
         S.streams.push_back(s);
         S.runners.push_back(std::make_unique<BatchRunner>());
         S.runner_bytes.push_back(0);
+        S.pin.push_back(nullptr);
+        S.pin_cap.push_back(0);
     }
     for (int w = 0; w < slots; w++) S.workers.emplace_back([&S, w] { S.worker(w); });
     *out = h.release();
@@ -468,6 +487,8 @@ int cm_server_destroy(cm_server h)
     S.cv_done.notify_all();
     for (auto &t : S.workers) t.join();
     for (auto s : S.streams) (void)hipStreamDestroy(s);
+    for (double *p : S.pin)
+        if (p) (void)hipHostFree(p);
     delete h;
     return CM_OK;
 }

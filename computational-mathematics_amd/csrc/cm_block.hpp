@@ -220,8 +220,12 @@ __global__ __launch_bounds__(BS) void spmm_short(int cnt, const int *__restrict_
 #pragma unroll
         for (int k = 0; k < KK; k++) acc[i][k] = 0.0;
     const int c0 = (threadIdx.x % LPRW) * CPL;
-    const int q = xcd_block(blockIdx.x, gridDim.x) * (BS / LPRW) + threadIdx.x / LPRW;
-    if (q < cnt) {
+    // a contiguous chunk of the row list per workgroup (XCD-remapped), walked
+    // BS / LPRW rows at a time: the grid is capped, so are the partials
+    constexpr int SR = BS / LPRW;
+    const int chunk = ((cnt + (int)gridDim.x - 1) / (int)gridDim.x + SR - 1) / SR * SR;
+    const int q_begin = xcd_block(blockIdx.x, gridDim.x) * chunk, q_end = min(cnt, q_begin + chunk);
+    for (int q = q_begin + threadIdx.x / LPRW; q < q_end; q += SR) {
         const int r = rows[q];
         Acc s[CPL];
 #pragma unroll
@@ -278,48 +282,89 @@ __global__ __launch_bounds__(BS) void spmm_long(int cnt, const int *__restrict__
         for (int k = 0; k < KK; k++) acc[i][k] = 0.0;
     const int lane = threadIdx.x & (WAVE - 1);
     const int c0 = (lane % LPRW) * CPL, g = lane / LPRW;
-    const int q = blockIdx.x * (BS / WAVE) + threadIdx.x / WAVE;
-    Acc s[CPL];
+    // a contiguous chunk of the row list per workgroup, one row per wave at a
+    // time (the loop bound is uniform over the wave: the shuffles below run
+    // on every lane)
+    constexpr int WPB = BS / WAVE;
+    const int chunk = ((cnt + (int)gridDim.x - 1) / (int)gridDim.x + WPB - 1) / WPB * WPB;
+    const int q_begin = blockIdx.x * chunk, q_end = min(cnt, q_begin + chunk);
+    for (int q0 = q_begin; q0 < q_end; q0 += WPB) {
+        const int q = q0 + threadIdx.x / WAVE;
+        Acc s[CPL];
 #pragma unroll
-    for (int i = 0; i < CPL; i++) s[i] = 0;
-    int r = -1;
-    if (q < cnt) {
-        r = rows[q];
-        const int e = ptr[r + 1];
-        int k = ptr[r] + g;
-        for (; k + 3 * G < e; k += 4 * G) {
-            int cc[4];
-            Acc vv[4];
+        for (int i = 0; i < CPL; i++) s[i] = 0;
+        int r = -1;
+        if (q < q_end) {
+            r = rows[q];
+            const int e = ptr[r + 1];
+            int k = ptr[r] + g;
+            for (; k + 3 * G < e; k += 4 * G) {
+                int cc[4];
+                Acc vv[4];
 #pragma unroll
-            for (int u = 0; u < 4; u++) {
-                cc[u] = col[k + u * G];
-                vv[u] = load_val<V, Acc>(val, k + u * G, false);
+                for (int u = 0; u < 4; u++) {
+                    cc[u] = col[k + u * G];
+                    vv[u] = load_val<V, Acc>(val, k + u * G, false);
+                }
+#pragma unroll
+                for (int u = 0; u < 4; u++) {
+                    T xv[CPL];
+                    ld_cols<T, CPL>(x + (size_t)cc[u] * KB + c0, xv);
+#pragma unroll
+                    for (int i = 0; i < CPL; i++) s[i] += vv[u] * (Acc)xv[i];
+                }
             }
-#pragma unroll
-            for (int u = 0; u < 4; u++) {
+            for (; k < e; k += G) {
                 T xv[CPL];
-                ld_cols<T, CPL>(x + (size_t)cc[u] * KB + c0, xv);
+                ld_cols<T, CPL>(x + (size_t)col[k] * KB + c0, xv);
+                const Acc vk = load_val<V, Acc>(val, k, false);
 #pragma unroll
-                for (int i = 0; i < CPL; i++) s[i] += vv[u] * (Acc)xv[i];
+                for (int i = 0; i < CPL; i++) s[i] += vk * (Acc)xv[i];
             }
         }
-        for (; k < e; k += G) {
-            T xv[CPL];
-            ld_cols<T, CPL>(x + (size_t)col[k] * KB + c0, xv);
-            const Acc vk = load_val<V, Acc>(val, k, false);
 #pragma unroll
-            for (int i = 0; i < CPL; i++) s[i] += vk * (Acc)xv[i];
+        for (int i = 0; i < CPL; i++)
+            for (int off = WAVE / 2; off >= LPRW; off >>= 1) s[i] += __shfl_xor(s[i], off, WAVE);
+        if (q < q_end && g == 0) {
+            const Acc sc = rowscale ? (Acc)rowscale[r] : (Acc)1;
+#pragma unroll
+            for (int i = 0; i < CPL; i++)
+                epi.row((size_t)r * KB + c0 + i, c0 + i, rowscale ? s[i] * sc : s[i], acc[i]);
         }
-    }
-#pragma unroll
-    for (int i = 0; i < CPL; i++)
-        for (int off = WAVE / 2; off >= LPRW; off >>= 1) s[i] += __shfl_xor(s[i], off, WAVE);
-    if (q < cnt && g == 0) {
-        const Acc sc = rowscale ? (Acc)rowscale[r] : (Acc)1;
-#pragma unroll
-        for (int i = 0; i < CPL; i++) epi.row((size_t)r * KB + c0 + i, c0 + i, rowscale ? s[i] * sc : s[i], acc[i]);
     }
     block_colgroup_partials<Epi::K, KB, CPL>(acc, red, partials, slot0 + blockIdx.x);
+}
+
+// First stage of a two-stage column reduction: workgroup g sums partial
+// slots [g * per, (g + 1) * per) of every column (same thread / column
+// layout and fixed order as k_block_finalize) into slot g of out.
+template <int K, int KB>
+__global__ __launch_bounds__(FBS) void k_block_sum1(int nblk, int per, const double *__restrict__ partials,
+                                                    double *__restrict__ out, const BlockFlags *fl)
+{
+    if (fl && fl->all_done) return;
+    __shared__ double red[(FBS / WAVE) * K * KB];
+    const int j = threadIdx.x % KB, lane = threadIdx.x & (WAVE - 1), w = threadIdx.x / WAVE;
+    const int b0 = blockIdx.x * per, b1 = min(nblk, b0 + per);
+    double s[K];
+#pragma unroll
+    for (int k = 0; k < K; k++) s[k] = 0.0;
+    for (int b = b0 + threadIdx.x / KB; b < b1; b += FBS / KB)
+#pragma unroll
+        for (int k = 0; k < K; k++) s[k] += partials[((size_t)b * K + k) * KB + j];
+#pragma unroll
+    for (int k = 0; k < K; k++)
+        for (int off = WAVE / 2; off >= KB; off >>= 1) s[k] += __shfl_xor(s[k], off, WAVE);
+    if (lane < KB)
+#pragma unroll
+        for (int k = 0; k < K; k++) red[(w * K + k) * KB + lane] = s[k];
+    __syncthreads();
+    if (threadIdx.x < KB * K) {
+        const int jj = threadIdx.x % KB, k = threadIdx.x / KB;
+        double q = 0.0;
+        for (int v = 0; v < FBS / WAVE; v++) q += red[(v * K + k) * KB + jj];
+        out[((size_t)blockIdx.x * K + k) * KB + jj] = q;
+    }
 }
 
 // One workgroup sums the nblk partials of every column at once: thread t

@@ -921,7 +921,7 @@ struct BlockKrylov {
       *t = nullptr, *pc = nullptr, *sc = nullptr, *tm = nullptr;
     SolveState *st = nullptr, *hst = nullptr;  // KB device states; pinned host copies (3 * KB)
     BlockFlags *fl = nullptr, *hfl = nullptr;  // device flags; pinned host copies (3)
-    double *partials = nullptr;
+    double *partials = nullptr, *partials2 = nullptr;  // slot partials; first-stage sums
     size_t partial_slots = 0;
     hipEvent_t ev[2] = {nullptr, nullptr};
     bool use_graph = true;
@@ -937,6 +937,7 @@ struct BlockKrylov {
         (void)hipFree(st);
         (void)hipFree(fl);
         (void)hipFree(partials);
+        (void)hipFree(partials2);
         if (hst) (void)hipHostFree(hst);
         if (hfl) (void)hipHostFree(hfl);
         for (auto &e : ev)
@@ -945,9 +946,30 @@ struct BlockKrylov {
 
     size_t ne() const { return (size_t)n * KB; }
     int vgrid() const { return (int)std::max<size_t>(1, std::min<size_t>((ne() + BS - 1) / BS, MAX_RED_BLOCKS)); }
-    static constexpr int SROWS = BS / ColSplit<T, KB>::LPRW;  // short rows per workgroup
-    static int short_grid(const RowBins &bn) { return (bn.ns + SROWS - 1) / SROWS; }
-    static int long_grid(const RowBins &bn) { return (bn.nl + (BS / WAVE) - 1) / (BS / WAVE); }
+
+    // per-column totals of nb partial slots -> fin on the states fst
+    template <int K, class Fin>
+    void finalize(int nb, Fin fin, SolveState *fst, bool gate, int update_all)
+    {
+        const double *src = partials;
+        if (nb > FIN_ONE) {
+            const int g1 = (nb + FIN_PER - 1) / FIN_PER;
+            k_block_sum1<K, KB><<<g1, FBS, 0, stream>>>(nb, FIN_PER, partials, partials2, gate ? fl : nullptr);
+            src = partials2;
+            nb = g1;
+        }
+        k_block_finalize<K, KB, Fin><<<1, FBS, 0, stream>>>(nb, src, fst, fl, fin, update_all);
+    }
+    // SpMM workgroups walk contiguous chunks of the row lists; the grid is
+    // one step per workgroup up to SPMM_CAP workgroups.  Column reductions
+    // over more than FIN_ONE partial slots go through a first stage of
+    // FIN_PER slots per workgroup (k_block_sum1), so the one-workgroup
+    // finalize always reads a few hundred slots at most.
+    static constexpr int SROWS = BS / ColSplit<T, KB>::LPRW;  // short rows per workgroup step
+    static constexpr int SPMM_CAP = 1 << 16;
+    static constexpr int FIN_ONE = 256, FIN_PER = 64;
+    static int short_grid(const RowBins &bn) { return std::min((bn.ns + SROWS - 1) / SROWS, SPMM_CAP); }
+    static int long_grid(const RowBins &bn) { return std::min((bn.nl + (BS / WAVE) - 1) / (BS / WAVE), SPMM_CAP); }
 
     // device bytes of a block solver of this type for an n-row system whose
     // three matrices have (short + long) row-bin workgroups `slots`
@@ -985,6 +1007,7 @@ struct BlockKrylov {
             slots = std::max(slots, (size_t)(short_grid(*bn) + long_grid(*bn)));
         partial_slots = std::max(slots, (size_t)MAX_RED_BLOCKS);
         CM_HIP(hipMalloc((void **)&partials, partial_slots * 4 * KB * sizeof(double)));
+        CM_HIP(hipMalloc((void **)&partials2, (partial_slots / FIN_PER + 1) * 4 * KB * sizeof(double)));
         for (auto &e : ev) CM_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
         CM_HIP(hipStreamSynchronize(stream));
         return CM_OK;
@@ -1004,8 +1027,7 @@ struct BlockKrylov {
         if (gl > 0)
             spmm_long<V, T, Acc, KB, Epi><<<gl, BS, 0, stream>>>(bn.nl, bn.longr, m.ptr, m.col, m.val, m.rowscale,
                                                                 xin, epi, gf, partials, gs);
-        if constexpr (Epi::K > 0)
-            k_block_finalize<Epi::K, KB, Epi><<<1, FBS, 0, stream>>>(gs + gl, partials, fst, fl, epi, gate ? 1 : 0);
+        if constexpr (Epi::K > 0) finalize<Epi::K>(gs + gl, epi, fst, gate, gate ? 1 : 0);
     }
 
     // Average time (microseconds) of `reps` back-to-back products y = M x of
@@ -1069,8 +1091,10 @@ struct BlockKrylov {
         if (slots <= partial_slots) return CM_OK;
         CM_HIP(hipStreamSynchronize(stream));
         (void)hipFree(partials);
-        partials = nullptr;
+        (void)hipFree(partials2);
+        partials = partials2 = nullptr;
         CM_HIP(hipMalloc((void **)&partials, slots * 4 * KB * sizeof(double)));
+        CM_HIP(hipMalloc((void **)&partials2, (slots / FIN_PER + 1) * 4 * KB * sizeof(double)));
         partial_slots = slots;
         return CM_OK;
     }
@@ -1109,7 +1133,7 @@ struct BlockKrylov {
         enqueue_apply(s, sc);
         spmm(*A, bA, sc, BEpiOmega<T>{t, s}, st);
         k_block_xr<T, KB><<<g, BS, 0, stream>>>(ne(), x, pc, sc, r, s, t, r0, st, fl, partials);
-        k_block_finalize<4, KB, FinPass><<<1, FBS, 0, stream>>>(g, partials, st, fl, FinPass{}, 1);
+        finalize<4>(g, FinPass{}, st, true, 1);
         k_block_p<T, KB><<<g, BS, 0, stream>>>(ne(), p, r, v, st, fl);
     }
 
@@ -1167,7 +1191,7 @@ struct BlockKrylov {
     {
         const int g = vgrid();
         k_block_setup_zero<T, KB><<<g, BS, 0, stream>>>(ne(), b, x, r, r0, p, partials);
-        k_block_finalize<1, KB, FinSetup><<<1, FBS, 0, stream>>>(g, partials, st, fl, FinSetup{}, 1);
+        finalize<1>(g, FinSetup{}, st, true, 1);
     }
 
     // passes until every column is done; fin receives the final states.

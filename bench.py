@@ -63,9 +63,9 @@ WORKLOADS = {
 CAT_NAMES = {"A": "A SpMV + fused dot(s)", "F1": "factor SpMV H (D^-1 fused)", "F2": "factor SpMV G",
              "LU": "L/U SpMV (mode C)", "VEC": "BiCGSTAB vector updates", "SETUP": "setup r=b-Ax",
              "FIN": "one-workgroup reduction finalize"}
-CAT_KERNELS = {"A": "spmv_kernel<double, double, cm::EpiAlpha|EpiOmega>",
-               "F1": "spmv_kernel<double, double, cm::EpiScale<double, double>>",
-               "F2": "spmv_kernel<double, double, cm::EpiStore<double>>"}
+CAT_KERNELS = {"A": "cm::spmv_kernel<2048, double, double, double, cm::EpiAlpha|EpiOmega<double> >",
+               "F1": "cm::spmv_kernel<2048, double, double, double, cm::EpiScale<double, double> >",
+               "F2": "cm::spmv_kernel<2048, double, double, double, cm::EpiStore<double> >"}
 
 
 def build_workload(w):
@@ -156,6 +156,8 @@ def bench_block(args, w):
           f"factors built in {t_build:.1f}s", file=sys.stderr, flush=True)
     sys_ = cmamd.System(w["prec"], A, F)
     sys_.enable_mixed(32)
+    if args.no_graphs:
+        sys_.block_graphs(False)
     steps = max(1, args.steps)
 
     def timed(fn, reps):
@@ -290,6 +292,9 @@ def bench_serve(args, w):
     print(f"[bench] workload c5: {len(systems)} systems (total n={n_tot}), built in {t_build:.1f}s; "
           f"{len(reqs)} requests at {args.c5_rate}/s", file=sys.stderr, flush=True)
     handles = [cmamd.System("afiim", A, F) for A, F in systems]
+    if args.no_graphs:
+        for h in handles:
+            h.block_graphs(False)
 
     def check(i, X):
         """(worst true rel-res, worst error vs x*) of request i's columns."""
@@ -457,8 +462,11 @@ def main():
     ap.add_argument("--c5-rate", type=float, default=200.0, help="Poisson arrival rate (requests/s)")
     ap.add_argument("--c5-slots", type=int, default=4)
     ap.add_argument("--c5-seed", type=int, default=5)
-    ap.add_argument("--c5-small-n", type=int, default=16384,
+    ap.add_argument("--c5-small-n", type=int, default=8192,
                     help="systems with at most this many rows go to the persistent batched kernel (0: never)")
+    ap.add_argument("--no-graphs", action="store_true",
+                    help="c4/c5: launch block passes kernel by kernel instead of replaying captured hipGraphs "
+                         "(for kernel tracers that do not follow graph launches)")
     ap.add_argument("--c5-fp64-requests", type=int, default=0,
                     help="replay only the first N requests on the FP64 one-at-a-time path (0: all)")
     ap.add_argument("--calibrate-copy", action="store_true",
@@ -527,6 +535,16 @@ def main():
     x = sys_.get_x()
     true_res = float(np.linalg.norm(b - cmamd.csr_mv_host(A, x)) / np.linalg.norm(b))
     fwd_err = float(np.linalg.norm(x - xs) / np.linalg.norm(xs))
+    # PCIe-inclusive rate: one call of the reference-signature drop-in entry
+    # point, which uploads A, the factors and b, solves, and copies x back
+    t0 = time.perf_counter()
+    if w["prec"] == "pdfiim":
+        dr = cmamd.pdfiim_epbicgstab(A, b, F, tol=w["tol"], nmax=w["nmax"])
+    elif w["prec"] == "afiim":
+        dr = cmamd.afiim_epbicgstab(A, b, F, tol=w["tol"], nmax=w["nmax"])
+    else:
+        dr = cmamd.acifim_pbicgstab(A, b, F, tol=w["tol"], nmax=w["nmax"])
+    t_dropin = time.perf_counter() - t0
     mixed = {}
     if not args.no_mixed:
         for bits in (32, 16):
@@ -569,6 +587,9 @@ def main():
         "iterations": int(np.median(iters)), "info": info, "resval": rv, "true_rel_residual": true_res,
         "forward_error_vs_xstar": fwd_err,
         "mixed_precision": mixed,
+        "pcie_inclusive": {"ms_per_solve": round(1e3 * t_dropin, 1), "iterations": dr.iter, "info": dr.info,
+                           "what": "one call of the drop-in entry point with host arrays: upload and check A, "
+                                   "the factors and b, solve, copy x back (not the bench value)"},
         "roofline": roofline, "kernels": kernels,
         "kernels_note": "per-kernel table from 2 extra solves with every launch bracketed by HIP events; "
                         "roofline from the timed region, where only the dominant kernel is bracketed",

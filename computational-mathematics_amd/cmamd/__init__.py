@@ -279,7 +279,7 @@ SOLVE_SYMBOLS = [
     "cm_system_spmv", "cm_system_apply", "cm_system_nnz", "cm_system_profile", "cm_system_kernel_stats",
     "cm_system_enable_mixed", "cm_system_solve_mixed",
     "cm_system_block_set", "cm_system_block_get", "cm_system_block_solve", "cm_system_block_solve_mixed",
-    "cm_system_block_graphs", "cm_system_block_release",
+    "cm_system_graphs", "cm_system_block_graphs", "cm_system_block_release",
     "cm_system_block_time_spmm", "cm_batch_solve", "cm_system_autotune", "cm_system_block_autotune", "cm_last_error", "cm_device_count",
     "cm_server_create", "cm_server_configure", "cm_server_destroy", "cm_server_add_system", "cm_server_submit", "cm_server_wait",
     "cm_server_stats",
@@ -321,6 +321,8 @@ def _type_solve(lib):
     lib.cm_system_block_solve_mixed.argtypes = [vp, C.c_int, C.c_double, C.c_double, C.c_int, C.c_int, PD, PI,
                                                 PI, PI]
     lib.cm_system_block_graphs.argtypes = [vp, C.c_int]
+    lib.cm_system_graphs.argtypes = [vp, C.c_int]
+    lib.cm_system_graphs.restype = C.c_int
     lib.cm_system_block_release.argtypes = [vp]
     lib.cm_system_block_time_spmm.argtypes = [vp, C.c_char, C.c_int, C.c_int, PD, PD]
     lib.cm_system_block_time_spmm.restype = C.c_int
@@ -462,6 +464,10 @@ class System:
             X0 = np.asfortranarray(np.asarray(X0, np.float64).reshape(self.n, self._bk))
             px = _dp(X0)
         _check(self._lib.cm_system_block_set(self._h, self._bk, _dp(B), px))
+
+    def graphs(self, on=True):
+        """Single-RHS passes from captured hipGraphs (on) or kernel by kernel."""
+        _check(self._lib.cm_system_graphs(self._h, 1 if on else 0))
 
     def block_graphs(self, on=True):
         _check(self._lib.cm_system_block_graphs(self._h, 1 if on else 0))

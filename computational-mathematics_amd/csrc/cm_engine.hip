@@ -359,6 +359,9 @@ static int max_div_steps(int variant, int n)
     return std::min(std::max(n, 500) / 50, 20);
 }
 
+// Passes per captured hipGraph (besides the one-pass graph).
+constexpr int GRAPH_PASSES = 4;
+
 // Kernel categories of the per-kernel timing (cm_system_kernel_stats).
 enum KernelCat { KC_A = 0, KC_F1 = 1, KC_F2 = 2, KC_LU = 3, KC_VEC = 4, KC_SETUP = 5, KC_FIN = 6, KC_N = 7 };
 
@@ -469,10 +472,22 @@ struct Krylov {
     int trace_cap_dev = 0;
     hipEvent_t ev[2] = {nullptr, nullptr};
     Prof prof;
+    // passes replayed from captured hipGraphs (1 pass, GRAPH_PASSES passes)
+    // when no launch is being bracketed by profiling events
+    bool use_graph = true;
+    hipGraphExec_t g1 = nullptr, gN = nullptr;
+
+    void drop_graphs()
+    {
+        if (g1) (void)hipGraphExecDestroy(g1);
+        if (gN) (void)hipGraphExecDestroy(gN);
+        g1 = gN = nullptr;
+    }
 
     ~Krylov()
     {
         if (stream) (void)hipStreamSynchronize(stream);
+        drop_graphs();
         T *bufs[] = {idv, b, x, r0, r, p, v, s, t, pc, sc, tm, t1, t2, t3};
         for (T *q : bufs) (void)hipFree(q);
         (void)hipFree(partials);
@@ -516,6 +531,7 @@ struct Krylov {
         const size_t need = (size_t)std::max(nblk, MAX_RED_BLOCKS) * 4;
         if (need <= partials_cap) return CM_OK;
         CM_HIP(hipStreamSynchronize(stream));
+        drop_graphs();  // their launches carry the old buffer
         (void)hipFree(partials);
         partials = nullptr;
         CM_HIP(hipMalloc((void **)&partials, need * sizeof(double)));
@@ -671,6 +687,33 @@ struct Krylov {
         prof.p_update = false;
     }
 
+    int capture(hipGraphExec_t &out, int passes)
+    {
+        hipGraph_t graph = nullptr;
+        CM_HIP(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+        for (int i = 0; i < passes; i++) enqueue_iteration();
+        CM_HIP(hipStreamEndCapture(stream, &graph));
+        const hipError_t err = hipGraphInstantiate(&out, graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        CM_HIP(err);
+        return CM_OK;
+    }
+
+    int enqueue_passes(int size)
+    {
+        if (prof.on || !use_graph) {
+            for (int i = 0; i < size; i++) enqueue_iteration();
+            return CM_OK;
+        }
+        if (!g1) {
+            CM_TRY(capture(g1, 1));
+            CM_TRY(capture(gN, GRAPH_PASSES));
+        }
+        for (; size >= GRAPH_PASSES; size -= GRAPH_PASSES) CM_HIP(hipGraphLaunch(gN, stream));
+        for (; size > 0; size--) CM_HIP(hipGraphLaunch(g1, stream));
+        return CM_OK;
+    }
+
     // Writes the initial state (done = 0) of a solve of NMAX passes.
     int begin(double tol, int NMAX, int info_in, int trace_cap)
     {
@@ -720,7 +763,7 @@ struct Krylov {
         int nq = 0, slot = 0, last_chunk = 4;
         long long enq = 0;
         auto push = [&](int size) -> int {
-            for (int i = 0; i < size; i++) enqueue_iteration();
+            CM_TRY(enqueue_passes(size));
             CM_HIP(hipMemcpyAsync(&hst[1 + slot], st, sizeof(SolveState), hipMemcpyDeviceToHost, stream));
             CM_HIP(hipEventRecord(ev[slot], stream));
             enq += size;
@@ -906,8 +949,6 @@ static int make_bins(RowBins &b, const DevCsr<V> &m, hipStream_t s, int short_ma
 // captured once into hipGraphs (kernel nodes only) and a chunk is a few graph
 // launches: the host cost of a pass drops from ~16 kernel launches to a
 // fraction of one graph launch, which is what small systems are bound by.
-constexpr int GRAPH_PASSES = 4;
-
 template <class T, class VA, class VF, int KB>
 struct BlockKrylov {
     int n = 0, variant = 0, ncols = 0;
@@ -1562,6 +1603,7 @@ struct System {
     std::unique_ptr<BlockBase> blks[5];
     BlockBase *blk = nullptr;
     bool block_graphs = true;
+    bool graphs = true;  // single-RHS passes from captured graphs
     size_t bytes() const
     {
         size_t b = k64->bytes() + (k32 ? k32->bytes() : 0) + (k16 ? k16->bytes() : 0);
@@ -1919,11 +1961,13 @@ int cm_system_enable_mixed(cm_system h, int factor_bits)
     if (factor_bits == 32 && !S.k32) {
         auto k = std::make_unique<K32>();
         CM_TRY(create_inner(*k, *S.k64));
+        k->use_graph = S.graphs;
         S.k32 = std::move(k);
     }
     if (factor_bits == 16 && !S.k16) {
         auto k = std::make_unique<K16>();
         CM_TRY(create_inner(*k, *S.k64));
+        k->use_graph = S.graphs;
         S.k16 = std::move(k);
     }
     return CM_OK;
@@ -1961,6 +2005,17 @@ int cm_system_block_set(cm_system h, int k, const double *B, const double *X0)
     S.blk->k16 = S.k16.get();
     S.blk->set_graphs(S.block_graphs);
     return S.blk->set(k, B, X0);
+}
+
+int cm_system_graphs(cm_system h, int on)
+{
+    if (!h) return fail(CM_ERR_ARG, "null handle");
+    System &S = h->sys;
+    S.k64->use_graph = on != 0;
+    if (S.k32) S.k32->use_graph = on != 0;
+    if (S.k16) S.k16->use_graph = on != 0;
+    S.graphs = on != 0;
+    return CM_OK;
 }
 
 int cm_system_block_graphs(cm_system h, int on)
@@ -2016,6 +2071,9 @@ int cm_system_autotune(cm_system h, int reps, int *tiles)
     }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
+    k.drop_graphs();  // their launches carry the old row blocks
+    if (S.k32) S.k32->drop_graphs();
+    if (S.k16) S.k16->drop_graphs();
     CM_TRY(k.ensure_partials(maxblk));
     if (S.k32) {
         K32 &c = *S.k32;

@@ -121,8 +121,12 @@ int cm_system_block_solve(cm_system sys, double tol, int NMAX, double *resval, i
 int cm_system_block_solve_mixed(cm_system sys, int factor_bits, double tol, double inner_tol, int NMAX,
                                 int max_outer, double *resval, int *outer, int *inner_total, int *info);
 int cm_system_block_get(cm_system sys, double *X);
-/* Block passes are replayed from hipGraphs captured on first use (on, the
- * default) or launched kernel by kernel (off); the results are identical. */
+/* BiCGSTAB passes are replayed from hipGraphs captured on first use (on, the
+ * default) or launched kernel by kernel (off); the results are identical.
+ * cm_system_graphs covers the single-RHS solves (which launch kernel by
+ * kernel anyway while cm_system_profile brackets launches), and
+ * cm_system_block_graphs the block solves. */
+int cm_system_graphs(cm_system sys, int on);
 int cm_system_block_graphs(cm_system sys, int on);
 /* Persistent batched solves: nitems right-hand sides of small resident
  * systems (item i: system systems[i], column B[i], initial guess and result

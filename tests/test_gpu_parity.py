@@ -261,3 +261,35 @@ def test_autotuned_tiles_keep_parity():
     sys_.close()
     assert info == 0
     assert np.linalg.norm(b - cmamd.csr_mv_host(A, xm)) / np.linalg.norm(b) <= 1e-8
+
+
+def test_graph_replays_match_eager_single():
+    """Single-RHS passes replayed from captured hipGraphs against kernel-by-
+    kernel launches: bit-identical traces and solutions over several replays,
+    with other eager work (a profiled solve, a mixed solve) in between."""
+    A = _system("c1")
+    b = cmamd.csr_mv_host(A, cmamd.gen_uniform(2, A.n))
+    F = cmamd.afiim(A, 0.1)
+    sys_ = cmamd.System("afiim", A, F)
+    sys_.enable_mixed(32)
+    sys_.set_rhs(b)
+    ref = None
+    for graphs in (False, True, True, True):
+        sys_.graphs(graphs)
+        sys_.set_x(None)
+        rv, it, info, tr = sys_.solve(tol=1e-8, nmax=500, trace_cap=64)
+        x = sys_.get_x()
+        sys_.set_x(None)
+        mx = sys_.solve_mixed(32, tol=1e-8, inner_tol=1e-4, nmax=500)
+        xm = sys_.get_x()
+        sys_.profile(True)  # eager, event-bracketed solve between replays
+        sys_.set_x(None)
+        sys_.solve(tol=1e-8, nmax=500)
+        sys_.profile(False)
+        got = (x, rv, it, info, tr, xm, mx)
+        if ref is None:
+            ref = got
+            continue
+        assert np.array_equal(got[0], ref[0]) and got[1:4] == ref[1:4] and np.array_equal(got[4], ref[4])
+        assert np.array_equal(got[5], ref[5]) and got[6] == ref[6]
+    sys_.close()

@@ -648,6 +648,9 @@ class Server:
         return sid.value
 
     def submit(self, sys_id, B, tol=1e-8, nmax=500) -> int:
+        if sys_id not in self._n:  # let the engine report the unknown id
+            rid = C.c_longlong()
+            _check(self._lib.cm_server_submit(self._h, sys_id, 1, _dp(np.zeros(1)), tol, nmax, C.byref(rid)))
         n = self._n[sys_id]
         B = np.asfortranarray(np.asarray(B, np.float64).reshape(n, -1))
         rid = C.c_longlong()

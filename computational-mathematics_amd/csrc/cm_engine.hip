@@ -775,19 +775,20 @@ struct Krylov {
         const long long nmax_passes = std::max(NMAX, 0);
         CM_TRY(push((int)std::min<long long>(4, std::max<long long>(nmax_passes, 1))));
         bool have_prev = false;
-        int prev_iter = 0;
-        double prev_rc = 0.0;
         for (;;) {
             if (nq < 2 && enq < nmax_passes) {
-                long long want = std::min(2 * last_chunk, 32);
+                long long want = std::min(2 * last_chunk, 16);
                 if (have_prev) {
+                    // passes still needed at the mean contraction rate since
+                    // the start, of which half are enqueued now: the chunks
+                    // shrink towards the exit, so few no-op passes run past it
                     const SolveState &sn = fin_state;  // latest snapshot
-                    if (sn.iter > prev_iter && sn.rc < prev_rc && sn.rc > 0.0 && prev_rc > 0.0) {
-                        const double rate = std::pow(sn.rc / prev_rc, 1.0 / (sn.iter - prev_iter));
-                        if (rate < 1.0) {
-                            const double need = std::ceil(std::log(sn.tol * sn.ro / sn.rc) / std::log(rate)) + 1.0;
+                    if (sn.iter >= 2 && sn.rc > 0.0 && sn.ro > 0.0 && sn.rc < sn.ro) {
+                        const double rate = std::pow(sn.rc / sn.ro, 1.0 / sn.iter);
+                        if (rate < 1.0 && rate > 0.0) {
+                            const double need = std::log(sn.tol * sn.ro / sn.rc) / std::log(rate);
                             const long long ahead = enq - sn.iter;
-                            want = (long long)std::max(2.0, std::min(32.0, need - (double)ahead));
+                            want = (long long)std::max(2.0, std::min(16.0, 0.5 * (need - (double)ahead)));
                         }
                     }
                 }
@@ -800,10 +801,6 @@ struct Krylov {
             nq--;
             CM_HIP(hipEventSynchronize(ev[o]));
             const SolveState snap = hst[1 + o];
-            if (have_prev || snap.iter > 0) {
-                prev_iter = have_prev ? fin_state.iter : 0;
-                prev_rc = have_prev ? fin_state.rc : snap.ro;
-            }
             fin_state = snap;
             have_prev = true;
             if (snap.done) break;
@@ -872,6 +869,17 @@ static int create_inner(KI &ki, const K64 &k)
     return CM_OK;
 }
 
+
+// Inner tolerance of one refinement step: inner_tol, or looser when the
+// residual rn only has to shrink by less than that to reach tol * res0 (half
+// of the remaining factor, at most 0.5).  The last step of a refinement then
+// stops as soon as the outer test can pass, instead of over-solving.
+static inline double adaptive_inner_tol(double inner_tol, double tol, double res0, double rn)
+{
+    if (!(rn > 0.0) || !(res0 > 0.0)) return inner_tol;
+    const double need = 0.5 * tol * res0 / rn;
+    return std::max(inner_tol, std::min(need, 0.5));
+}
 
 // ------------------------------------------------------------- autotuner
 //
@@ -1207,12 +1215,12 @@ struct BlockKrylov {
     }
 
     // initial per-column states; columns not in `active` start done
-    int begin(double tol, int NMAX, const int *active)
+    int begin(double tol, int NMAX, const int *active, const double *col_tol = nullptr)
     {
         for (int j = 0; j < KB; j++) {
             SolveState &q = hst[j];
             std::memset(&q, 0, sizeof q);
-            q.tol = tol;
+            q.tol = (col_tol && j < ncols) ? col_tol[j] : tol;
             q.eps = eps;
             q.nmax = NMAX;
             q.maxmsteps = max_div_steps(variant, n);
@@ -1275,7 +1283,7 @@ struct BlockKrylov {
                 if (rate < 1.0 && rate > 0.0) need = std::max(need, std::log(q.tol * q.ro / q.rc) / std::log(rate));
             }
             const long long ahead = enq - hst[KB * (1 + o)].iter;  // passes already queued beyond this snapshot
-            chunk = (int)std::min<double>(32.0, std::max<double>(2.0, need - (double)std::max<long long>(ahead, 0)));
+            chunk = (int)std::min<double>(16.0, std::max<double>(2.0, 0.5 * (need - (double)std::max<long long>(ahead, 0))));
         }
         CM_HIP(hipStreamSynchronize(stream));
         CM_HIP(hipGetLastError());
@@ -1454,7 +1462,10 @@ struct BlockSolver : BlockBase {
             }
             steps = o;
             if (!any) break;
-            CM_TRY(bi.begin(inner_tol, NMAX, act.data()));
+            std::vector<double> itol(ncols, inner_tol);
+            for (int j = 0; j < ncols; j++)
+                if (act[j]) itol[j] = adaptive_inner_tol(inner_tol, tol, res0[j], host_ost[j].res_outer);
+            CM_TRY(bi.begin(inner_tol, NMAX, act.data(), itol.data()));
             bi.setup_zero();
             SolveState fin[KB];
             CM_TRY(bi.run(NMAX, fin));
@@ -1736,7 +1747,7 @@ static int solve_mixed(System &S, KI &ki, double tol, double inner_tol, int NMAX
             return CM_OK;
         }
         prev = rn;
-        CM_TRY(ki.begin(inner_tol, NMAX, 0, 0));
+        CM_TRY(ki.begin(adaptive_inner_tol(inner_tol, tol, res0, rn), NMAX, 0, 0));
         ki.setup_zero();
         SolveState fin;
         CM_TRY(ki.run(NMAX, fin));

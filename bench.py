@@ -95,19 +95,27 @@ def cpu_baseline(w, A, b, F, seconds):
     threads = max(1, min(threads, 64))
     oracle.set_threads(threads)
     times, its = [], []
+    x_or = None
     t_start = time.perf_counter()
     while True:
         t0 = time.perf_counter()
-        _, _, it, info, _ = oracle.bicgstab(w["prec"], A, b, F, tol=w["tol"], nmax=w["nmax"])
+        x, _, it, info, _ = oracle.bicgstab(w["prec"], A, b, F, tol=w["tol"], nmax=w["nmax"])
         times.append(time.perf_counter() - t0)
         its.append(it)
+        if x_or is None:
+            x_or = x
         if time.perf_counter() - t_start >= seconds or len(times) >= 50:
             break
     med = float(np.median(times))
     return {"value": 1.0 / med, "unit": "solves/s", "cores": threads, "kind": "port",
             "sample": f"{len(times)} full solve(s) of this workload ({its[0]} iterations each) by the oracle "
                       f"(C restatement of the reference loop, OpenMP, {threads} threads); median "
-                      f"{med * 1e3:.1f} ms/solve"}
+                      f"{med * 1e3:.1f} ms/solve",
+            "iterations": its[0]}, x_or
+
+
+def _rel(a, b):
+    return float(np.linalg.norm(a - b) / np.linalg.norm(b))
 
 
 def mixed_leg(sys_, bits, w, A, b, x64, args):
@@ -236,7 +244,8 @@ def bench_block(args, w):
                         "H": {"avg_us": round(us_h, 2), "GBs": round(by_h / us_h / 1e3, 1)}},
     }
     if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(w, A, B[:, 0], F, args.cpu_seconds)
+        out["cpu_baseline"], x_or = cpu_baseline(w, A, B[:, 0], F, args.cpu_seconds)
+        out["rel_error_vs_oracle_column0"] = {"mixed_block": _rel(Xmx[:, 0], x_or), "fp64_single": _rel(X64[:, 0], x_or)}
     else:
         out["cpu_baseline"] = None
     print(json.dumps(out), flush=True)
@@ -546,9 +555,11 @@ def main():
         dr = cmamd.acifim_pbicgstab(A, b, F, tol=w["tol"], nmax=w["nmax"])
     t_dropin = time.perf_counter() - t0
     mixed = {}
+    xs_mixed = {}
     if not args.no_mixed:
         for bits in (32, 16):
             mixed[f"fp{bits}_factors"] = mixed_leg(sys_, bits, w, A, b, x, args)
+            xs_mixed[f"fp{bits}_factors"] = sys_.get_x()
     sys_.close()
 
     value = args.steps / elapsed
@@ -595,7 +606,9 @@ def main():
                         "roofline from the timed region, where only the dominant kernel is bracketed",
     }
     if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(w, A, b, F, args.cpu_seconds)
+        out["cpu_baseline"], x_or = cpu_baseline(w, A, b, F, args.cpu_seconds)
+        # full-size parity: the GPU solutions against the oracle's FP64 solution of the same system
+        out["rel_error_vs_oracle"] = {"fp64": _rel(x, x_or), **{k: _rel(v, x_or) for k, v in xs_mixed.items()}}
     else:
         out["cpu_baseline"] = None
     print(json.dumps(out), flush=True)

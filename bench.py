@@ -473,6 +473,7 @@ def main():
     ap.add_argument("--c5-seed", type=int, default=5)
     ap.add_argument("--c5-small-n", type=int, default=8192,
                     help="systems with at most this many rows go to the persistent batched kernel (0: never)")
+    ap.add_argument("--no-autotune", action="store_true", help="keep the default SpMV row blocks and kernels")
     ap.add_argument("--no-graphs", action="store_true",
                     help="c4/c5: launch block passes kernel by kernel instead of replaying captured hipGraphs "
                          "(for kernel tracers that do not follow graph launches)")
@@ -511,6 +512,14 @@ def main():
           f"nnz(H)={F.H.nnz} factors built in {t_build:.1f}s", file=sys.stderr, flush=True)
 
     sys_ = cmamd.System(w["prec"], A, F)
+    tuned = None
+    if not args.no_autotune:
+        # per-matrix SpMV row-block size, and for the low-precision copies the
+        # 16-bit-index product against the 32-bit one, chosen by timing
+        if not args.no_mixed:
+            sys_.enable_mixed(32)
+            sys_.enable_mixed(16)
+        tuned = sys_.autotune(reps=10)
     sys_.set_rhs(b)
     for _ in range(args.warmup):
         sys_.set_x(None)
@@ -576,7 +585,12 @@ def main():
     n_l, ms, by = dstats
     d_avg_us = 1e3 * ms / n_l
     d_gbs = by / ms / 1e6
-    roofline = {"bound": "hbm", "kernel": CAT_KERNELS.get(dom, dom), "achieved": round(d_gbs, 1),
+    kname = CAT_KERNELS.get(dom, dom)
+    if tuned:
+        tile = {"A": tuned["A"], "F1": tuned["GT"] or tuned["H"], "F2": tuned["G"]}.get(dom)
+        if tile:
+            kname = kname.replace("<2048,", f"<{tile},")
+    roofline = {"bound": "hbm", "kernel": kname, "achieved": round(d_gbs, 1),
                 "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(d_gbs / HBM_PEAK_GBS, 4),
                 "traffic": None, "bytes_per_launch": by / n_l, "avg_us": round(d_avg_us, 2),
                 "launches_timed": n_l}
@@ -597,7 +611,7 @@ def main():
                        "p99": round(1e3 * float(np.percentile(lat, 99)), 3)},
         "iterations": int(np.median(iters)), "info": info, "resval": rv, "true_rel_residual": true_res,
         "forward_error_vs_xstar": fwd_err,
-        "mixed_precision": mixed,
+        "mixed_precision": mixed, "autotune_tiles": tuned,
         "pcie_inclusive": {"ms_per_solve": round(1e3 * t_dropin, 1), "iterations": dr.iter, "info": dr.info,
                            "what": "one call of the drop-in entry point with host arrays: upload and check A, "
                                    "the factors and b, solve, copy x back (not the bench value)"},

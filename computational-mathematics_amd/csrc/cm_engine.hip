@@ -179,6 +179,9 @@ struct DevCsr {
     int tile = TILE;  // staged nonzeros per SpMV workgroup the row blocks were cut for
     bool owns_index = true;
     int *ptr = nullptr, *col = nullptr, *blk = nullptr;
+    uint16_t *d16 = nullptr;  // optional block-relative 16-bit columns (col - cbase[block])
+    int *cbase = nullptr;
+    bool use16 = false;       // low-precision copy: products through spmv_vl16
     V *val = nullptr;
     float *rowscale = nullptr;
     int nlev = 0;
@@ -190,7 +193,10 @@ struct DevCsr {
     {
         if (!present()) return 0;
         size_t b = (size_t)nnz * sizeof(V) + (rowscale ? (size_t)n * sizeof(float) : 0);
-        if (owns_index) b += (size_t)(n + 1 + nblk + 1) * sizeof(int) + (size_t)nnz * sizeof(int) + (lrows ? (size_t)n * sizeof(int) : 0);
+        if (owns_index) {
+            b += (size_t)(n + 1 + nblk + 1) * sizeof(int) + (size_t)nnz * sizeof(int) + (lrows ? (size_t)n * sizeof(int) : 0);
+            if (d16) b += (size_t)(nnz + 2 * CSR_PAD) * sizeof(uint16_t) + (size_t)nblk * sizeof(int);
+        }
         return b;
     }
     void release()
@@ -200,10 +206,13 @@ struct DevCsr {
             (void)hipFree(col);
             (void)hipFree(blk);
             (void)hipFree(lrows);
+            (void)hipFree(d16);
+            (void)hipFree(cbase);
         }
         (void)hipFree(val);
         (void)hipFree(rowscale);
-        ptr = col = blk = lrows = nullptr;
+        ptr = col = blk = lrows = cbase = nullptr;
+        d16 = nullptr;
         val = nullptr;
         rowscale = nullptr;
     }
@@ -249,6 +258,11 @@ static int cast_csr(DevCsr<V> &dst, const DevCsr<double> &src, hipStream_t s)
     dst.nnz = src.nnz;
     dst.nblk = src.nblk;
     dst.tile = src.tile;
+    dst.d16 = src.d16;
+    dst.cbase = src.cbase;
+    // the staged 16-bit path pays off from a few nonzeros per row up (the
+    // autotuner settles it per matrix by timing)
+    dst.use16 = src.d16 && src.nnz >= 4LL * src.n;
     dst.owns_index = false;
     dst.ptr = src.ptr;
     dst.col = src.col;
@@ -265,6 +279,42 @@ static int cast_csr(DevCsr<V> &dst, const DevCsr<double> &src, hipStream_t s)
         k_cast<V><<<long_grid(dst.nnz), BS, 0, s>>>(dst.nnz, src.val, dst.val);
     }
     CM_HIP(hipGetLastError());
+    return CM_OK;
+}
+
+// Block-relative 16-bit column indices of a resident FP64 matrix for the
+// current row blocks (for the low-precision copies, which share them): each
+// block's columns as offsets from its smallest column.  Left absent when a
+// block's columns span 65536 or more.
+static int build_idx16(DevCsr<double> &m, hipStream_t s)
+{
+    if (!m.present()) return CM_OK;
+    (void)hipFree(m.d16);
+    (void)hipFree(m.cbase);
+    m.d16 = nullptr;
+    m.cbase = nullptr;
+    std::vector<int> ptr(m.n + 1), blk(m.nblk + 1), col(m.nnz);
+    CM_HIP(hipMemcpyAsync(ptr.data(), m.ptr, ptr.size() * sizeof(int), hipMemcpyDeviceToHost, s));
+    CM_HIP(hipMemcpyAsync(blk.data(), m.blk, blk.size() * sizeof(int), hipMemcpyDeviceToHost, s));
+    if (m.nnz) CM_HIP(hipMemcpyAsync(col.data(), m.col, col.size() * sizeof(int), hipMemcpyDeviceToHost, s));
+    CM_HIP(hipStreamSynchronize(s));
+    std::vector<int> cb(m.nblk, 0);
+    std::vector<uint16_t> d((size_t)m.nnz + 2 * CSR_PAD, 0);
+    for (int b = 0; b < m.nblk; b++) {
+        const long long k0 = ptr[blk[b]], k1 = ptr[blk[b + 1]];
+        int lo = INT32_MAX, hi = -1;
+        for (long long k = k0; k < k1; k++) {
+            lo = std::min(lo, col[k]);
+            hi = std::max(hi, col[k]);
+        }
+        if (hi < 0) continue;
+        if (hi - lo > 65535) return CM_OK;  // keep the 32-bit indices
+        cb[b] = lo;
+        for (long long k = k0; k < k1; k++) d[k] = (uint16_t)(col[k] - lo);
+    }
+    CM_TRY(dev_upload(&m.d16, d.data(), d.size(), s));
+    CM_TRY(dev_upload(&m.cbase, cb.data(), cb.size(), s));
+    CM_HIP(hipStreamSynchronize(s));
     return CM_OK;
 }
 
@@ -570,8 +620,9 @@ struct Krylov {
     template <class V, class Epi>
     double spmv_bytes(const DevCsr<V> &m) const
     {
-        return (double)m.nnz * (sizeof(V) + sizeof(int)) + 4.0 * (m.n + 1) + 4.0 * (m.nblk + 1) +
-               (m.rowscale ? 4.0 * m.n : 0.0) + (double)sizeof(T) * m.n * (2 + Epi::XV);
+        const bool i16 = m.d16 && m.use16 && !std::is_same<V, double>::value;
+        return (double)m.nnz * (sizeof(V) + (i16 ? 2.0 : 4.0)) + 4.0 * (m.n + 1) + 4.0 * (m.nblk + 1) +
+               (i16 ? 4.0 * m.nblk : 0.0) + (m.rowscale ? 4.0 * m.n : 0.0) + (double)sizeof(T) * m.n * (2 + Epi::XV);
     }
 
     template <class V, class Epi>
@@ -583,6 +634,20 @@ struct Krylov {
         SolveState *kst = fst ? fst : st;
         SolveState *gate = use_state ? kst : nullptr;
         timed(cat, spmv_bytes<V, Epi>(m), [&] {
+            if constexpr (!std::is_same<V, double>::value) {
+                if (m.d16 && m.use16) {  // low precision with 16-bit indices
+                    if (m.tile == 1024)
+                        spmv_vl16<1024, V, T, Acc, Epi><<<m.nblk, BS, 0, stream>>>(
+                            m.ptr, m.d16, m.cbase, m.val, m.rowscale, m.blk, xin, epi, gate, partials);
+                    else if (m.tile == 4096)
+                        spmv_vl16<4096, V, T, Acc, Epi><<<m.nblk, BS, 0, stream>>>(
+                            m.ptr, m.d16, m.cbase, m.val, m.rowscale, m.blk, xin, epi, gate, partials);
+                    else
+                        spmv_vl16<2048, V, T, Acc, Epi><<<m.nblk, BS, 0, stream>>>(
+                            m.ptr, m.d16, m.cbase, m.val, m.rowscale, m.blk, xin, epi, gate, partials);
+                    return;
+                }
+            }
             if constexpr (std::is_same<V, __half>::value) {
                 // 2-byte values: the vector-load variant (fewer load instructions)
                 if (m.tile == 1024)
@@ -852,8 +917,11 @@ static int create_k64(K64 &k, const HostSys &h, hipStream_t strm)
 
 // Low-precision inner solver cast from the resident FP64 one.
 template <class KI>
-static int create_inner(KI &ki, const K64 &k)
+static int create_inner(KI &ki, K64 &k)
 {
+    // 16-bit indices for the low-precision products (built once, shared)
+    for (DevCsr<double> *m : {&k.A, &k.G, &k.H, &k.GT, &k.L, &k.U})
+        if (m->present() && !m->d16) CM_TRY(build_idx16(*m, k.stream));
     ki.copy_shape(k.variant, k.n, k.nr, k.mode, k.sym, k.stream);
     ki.eps = 5.9604644775390625e-08;  // 2^-24: the stagnation test at FP32 resolution
     CM_TRY(cast_csr(ki.A, k.A, k.stream));
@@ -911,6 +979,47 @@ static void follow_blocks(DevCsr<V> &dst, const DevCsr<double> &src)
     dst.blk = src.blk;
     dst.nblk = src.nblk;
     dst.tile = src.tile;
+    dst.d16 = src.d16;
+    dst.cbase = src.cbase;
+    if (!dst.d16) dst.use16 = false;
+}
+
+// Autotuner, low-precision copies: per matrix, the staged 16-bit-index
+// product against the 32-bit one (reps products each), the faster kept.
+template <class KI>
+static int tune_low(KI &ki, int reps, hipEvent_t e0, hipEvent_t e1)
+{
+    using VF = typename std::remove_reference<decltype(ki.G)>::type;
+    auto one = [&](auto &m) -> int {
+        if (!m.present() || !m.d16) return CM_OK;
+        float best = 1e30f;
+        bool pick = m.use16;
+        for (bool u : {false, true}) {
+            m.use16 = u;
+            ki.spmv(m, ki.x, EpiStore<float>{ki.t}, false, KC_A);
+            CM_HIP(hipEventRecord(e0, ki.stream));
+            for (int i = 0; i < reps; i++) ki.spmv(m, ki.x, EpiStore<float>{ki.t}, false, KC_A);
+            CM_HIP(hipEventRecord(e1, ki.stream));
+            CM_HIP(hipEventSynchronize(e1));
+            float ms = 0.f;
+            CM_HIP(hipEventElapsedTime(&ms, e0, e1));
+            if (ms < best) {
+                best = ms;
+                pick = u;
+            }
+        }
+        m.use16 = pick;
+        return CM_OK;
+    };
+    (void)sizeof(VF);
+    CM_TRY(one(ki.A));
+    CM_TRY(one(ki.G));
+    CM_TRY(one(ki.H));
+    CM_TRY(one(ki.GT));
+    CM_TRY(one(ki.L));
+    CM_TRY(one(ki.U));
+    ki.drop_graphs();
+    return CM_OK;
 }
 
 // ------------------------------------------------------- block (multi-RHS)
@@ -2076,7 +2185,9 @@ int cm_system_autotune(cm_system h, int reps, int *tiles)
                 best_t = tile;
             }
         }
+        const bool had16 = m.d16 != nullptr;
         CM_TRY(retile(m, best_t, k.stream));
+        if (had16) CM_TRY(build_idx16(m, k.stream));  // relative to the new row blocks
         maxblk = std::max(maxblk, m.nblk);
         if (tiles) tiles[q] = best_t;
     }
@@ -2086,18 +2197,25 @@ int cm_system_autotune(cm_system h, int reps, int *tiles)
     if (S.k32) S.k32->drop_graphs();
     if (S.k16) S.k16->drop_graphs();
     CM_TRY(k.ensure_partials(maxblk));
+    hipEvent_t f0, f1;
+    CM_HIP(hipEventCreate(&f0));
+    CM_HIP(hipEventCreate(&f1));
     if (S.k32) {
         K32 &c = *S.k32;
         follow_blocks(c.A, k.A), follow_blocks(c.G, k.G), follow_blocks(c.H, k.H);
         follow_blocks(c.GT, k.GT), follow_blocks(c.L, k.L), follow_blocks(c.U, k.U);
         CM_TRY(c.ensure_partials(maxblk));
+        CM_TRY(tune_low(c, reps, f0, f1));
     }
     if (S.k16) {
         K16 &c = *S.k16;
         follow_blocks(c.A, k.A), follow_blocks(c.G, k.G), follow_blocks(c.H, k.H);
         follow_blocks(c.GT, k.GT), follow_blocks(c.L, k.L), follow_blocks(c.U, k.U);
         CM_TRY(c.ensure_partials(maxblk));
+        CM_TRY(tune_low(c, reps, f0, f1));
     }
+    (void)hipEventDestroy(f0);
+    (void)hipEventDestroy(f1);
     CM_HIP(hipGetLastError());
     return CM_OK;
 }

@@ -293,3 +293,32 @@ def test_graph_replays_match_eager_single():
         assert np.array_equal(got[0], ref[0]) and got[1:4] == ref[1:4] and np.array_equal(got[4], ref[4])
         assert np.array_equal(got[5], ref[5]) and got[6] == ref[6]
     sys_.close()
+
+
+def test_mixed_with_wide_column_span():
+    """Low-precision products use block-relative 16-bit column indices when a
+    row block's columns span less than 65536; a matrix with couplings 70,000
+    columns apart keeps 32-bit indices for the blocks' matrix.  Both must give
+    the same refinement result quality."""
+    import scipy.sparse as sp
+    base = cmamd.gen_convdiff(2, 290, 1.0, seed=61).to_scipy().tolil()  # n = 84100
+    n = base.shape[0]
+    rng = np.random.default_rng(5)
+    for i in rng.choice(n - 70000, size=40, replace=False):
+        base[i, i + 70000] = -0.01
+        base[i + 70000, i] = -0.01
+    A = cmamd.Csr.from_scipy(sp.csr_matrix(base))
+    F = cmamd.afiim(A, 0.1)
+    b = cmamd.csr_mv_host(A, cmamd.gen_uniform(62, n))
+    for bits in (32, 16):
+        sys_ = cmamd.System("afiim", A, F)
+        sys_.enable_mixed(bits)
+        sys_.set_rhs(b)
+        sys_.set_x(None)
+        rv, outer, inner, info = sys_.solve_mixed(bits, tol=1e-8, inner_tol=1e-4, nmax=2000)
+        x = sys_.get_x()
+        sys_.close()
+        assert info == 0
+        assert np.linalg.norm(b - cmamd.csr_mv_host(A, x)) / np.linalg.norm(b) <= 1e-8
+        xo = oracle.bicgstab("afiim", A, b, F, tol=1e-8, nmax=2000)[0]
+        assert _rel(x, xo) <= 1e-4

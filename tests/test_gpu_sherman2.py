@@ -117,3 +117,27 @@ def test_dropin_example_binary(tmp_path):
     x = cmamd.read_mtx_vector(str(px))
     xo = oracle.bicgstab("afiim", A, b, _golden_afiim(), nmax=500)[0]
     assert np.linalg.norm(x - xo) / np.linalg.norm(xo) < 1e-4
+
+
+def test_block_and_batch_on_sherman2():
+    """The reference fixture through the block path (3 copies of the
+    reference right-hand side plus 2 scaled ones) and the persistent batched
+    kernel, held to the same bars as the single solves on this matrix."""
+    A, b = _sherman2()
+    F = _golden_afiim()
+    B = np.stack([b, b, b, 2.0 * b, -0.5 * b], axis=1)
+    sys_ = cmamd.System("afiim", A, F)
+    sys_.block_set(B)
+    rv, it, info = sys_.block_solve(tol=1e-8, nmax=500)
+    X = sys_.block_get()
+    res = cmamd.batch_solve([(sys_, B[:, j], None, 1e-8, 500) for j in range(B.shape[1])])
+    sys_.close()
+    xo, rvo, ito, info_o, _ = oracle.bicgstab("afiim", A, b, F, tol=1e-8, nmax=500)
+    for j in range(B.shape[1]):
+        scale = B[0, j] / b[0]
+        assert info[j] == info_o == 0 and abs(it[j] - ito) <= 3
+        assert np.linalg.norm(X[:, j] - scale * xo) / np.linalg.norm(scale * xo) < 1e-4
+        assert res[j].info == 0 and abs(res[j].iter - ito) <= 3
+        assert np.linalg.norm(res[j].x - scale * xo) / np.linalg.norm(scale * xo) < 1e-4
+    # identical columns of one block give identical results
+    assert np.array_equal(X[:, 0], X[:, 1]) and np.array_equal(X[:, 1], X[:, 2])

@@ -32,29 +32,29 @@ HBM_PEAK_GBS = 8000.0
 METRIC = "PBiCGSTAB solves/s + p99 latency at rel-res 1e-8; SpMV/M-apply HBM GB/s vs peak"
 
 WORKLOADS = {
-    "c2": dict(
+    "cd3d-128": dict(
         desc="3D 7-point convection-diffusion 128^3 (n=2097152, nnz=14581760), log-normal diffusion, "
              "velocity (1,1,1), cell Peclet 1; PDFIIM(prefill=2, lfill=2, postfill=1, eps=0.05); "
              "b=A x*, x*~U[-1,1], x0=0; plain FP64 BiCGSTAB to rel-res 1e-8",
         dim=3, N=128, pe=1.0, seed=2, xseed=3, prec="pdfiim", args=(2, 2, 1, 0.05), tol=1e-8, nmax=1200),
-    "c3": dict(
+    "cd3d-128-pe5": dict(
         desc="3D 7-point convection-diffusion 128^3 (n=2097152, nnz=14581760), cell Peclet 5; "
              "ACIFIM(eps1=0.01, eps2=0.1), apply mode E; b=A x*, x0=0; plain FP64 BiCGSTAB to rel-res 1e-10",
         dim=3, N=128, pe=5.0, seed=4, xseed=5, prec="acifim", args=(0.01, 0.1), tol=1e-10, nmax=2000),
-    "c4": dict(
+    "st27-100-rhs16": dict(
         desc="3D 27-point stencil 100^3 (n=1000000), off-diagonals -(1+0.3 U[-1,1]), diagonal 1.05 x row "
              "absolute sum, 5% of rows with 200-600 long-range couplings -0.01 U[0,1] (nnz~46.3M); AFIIM "
              "eps=0.05; block of 16 RHS b_j=A x*_j, x0=0; FP32 inner block BiCGSTAB (FP32 A and factors, "
              "to 1e-4) inside per-column FP64 refinement to true rel-res 1e-8",
         gen="stencil27", N=100, frac=0.05, lo=200, hi=600, seed=11, xseed=12, prec="afiim", args=(0.05,),
         tol=1e-8, nmax=500, k=16),
-    "c5": dict(
+    "serve-mix": dict(
         desc="serving: Poisson arrivals, 2D 5-point / 3D 7-point convection-diffusion systems (50/50), n "
              "log-uniform in [1e3, 5e5], log-normal diffusion, Peclet U[0,5], AFIIM eps=0.1 resident; 1-8 RHS "
              "per request; FP32-inner / FP64-refinement to true rel-res 1e-8 with continuous batching, vs the "
              "plain FP64 one-at-a-time GPU path on the same requests",
         prec="afiim", tol=1e-8, nmax=1000),
-    "c1": dict(
+    "cd2d-100": dict(
         desc="2D 5-point convection-diffusion 100^2 (n=10000, nnz=49600), cell Peclet 0.5; AFIIM eps=0.1; "
              "b=A x*, x0=0; plain FP64 BiCGSTAB to rel-res 1e-8",
         dim=2, N=100, pe=0.5, seed=1, xseed=2, prec="afiim", args=(0.1,), tol=1e-8, nmax=500),
@@ -152,7 +152,7 @@ def _pct(v, q):
 
 
 def bench_block(args, w):
-    """C4: one step = one block solve of k right-hand sides (FP32 inner /
+    """Block workload: one step = one block solve of k right-hand sides (FP32 inner /
     FP64 refinement per column).  Baselines on the same inputs: the plain
     FP64 GPU path one column at a time, and the FP64 block solve."""
     import cmamd
@@ -160,7 +160,7 @@ def bench_block(args, w):
     k = w["k"]
     XS = np.stack([cmamd.gen_uniform(w["xseed"] + j, A.n) for j in range(k)], axis=1)
     B = np.stack([cmamd.csr_mv_host(A, XS[:, j]) for j in range(k)], axis=1)
-    print(f"[bench] workload c4: n={A.n} nnz(A)={A.nnz} nnz(G)={F.G.nnz} nnz(H)={F.H.nnz} "
+    print(f"[bench] workload st27-100-rhs16: n={A.n} nnz(A)={A.nnz} nnz(G)={F.G.nnz} nnz(H)={F.H.nnz} "
           f"factors built in {t_build:.1f}s", file=sys.stderr, flush=True)
     sys_ = cmamd.System(w["prec"], A, F)
     sys_.enable_mixed(32)
@@ -251,7 +251,7 @@ def bench_block(args, w):
     print(json.dumps(out), flush=True)
 
 
-def _c5_systems(nsys, seed):
+def _serve_systems(nsys, seed):
     """nsys systems: kind 2D/3D 50/50, n log-uniform in [1e3, 5e5], N =
     round(n^(1/d)), Peclet U[0,5], log-normal diffusion; AFIIM eps=0.1 built
     on host threads."""
@@ -274,32 +274,32 @@ def _c5_systems(nsys, seed):
 
 
 def bench_serve(args, w):
-    """C5: Poisson arrivals of solve requests over resident systems.  The
+    """Serving workload: Poisson arrivals of solve requests over resident systems.  The
     served path (continuous batching, FP32 inner / FP64 refinement) and the
     plain FP64 one-at-a-time path replay the same request list on the same
     arrival clock; latency = arrival to solution on the host."""
     import threading
     import cmamd
-    rng = np.random.default_rng(args.c5_seed)
+    rng = np.random.default_rng(args.serve_seed)
     t0 = time.perf_counter()
-    systems = _c5_systems(args.c5_systems, args.c5_seed)
+    systems = _serve_systems(args.serve_systems, args.serve_seed)
     t_build = time.perf_counter() - t0
     # per system a pool of 8 right-hand sides b = A x*
     pools = []
     for s, (A, _) in enumerate(systems):
         XS = np.stack([cmamd.gen_uniform(50000 + 8 * s + j, A.n) for j in range(8)], axis=1)
         pools.append((XS, np.stack([cmamd.csr_mv_host(A, XS[:, j]) for j in range(8)], axis=1)))
-    gaps = rng.exponential(1.0 / args.c5_rate, args.c5_requests)
+    gaps = rng.exponential(1.0 / args.serve_rate, args.serve_requests)
     arrive = np.cumsum(gaps)
     reqs = []
-    for i in range(args.c5_requests):
+    for i in range(args.serve_requests):
         s = int(rng.integers(len(systems)))
         k = int(rng.integers(1, 9))
         cols = np.sort(rng.choice(8, size=k, replace=False))
         reqs.append((s, cols))
     n_tot = sum(A.n for A, _ in systems)
-    print(f"[bench] workload c5: {len(systems)} systems (total n={n_tot}), built in {t_build:.1f}s; "
-          f"{len(reqs)} requests at {args.c5_rate}/s", file=sys.stderr, flush=True)
+    print(f"[bench] workload serve-mix: {len(systems)} systems (total n={n_tot}), built in {t_build:.1f}s; "
+          f"{len(reqs)} requests at {args.serve_rate}/s", file=sys.stderr, flush=True)
     handles = [cmamd.System("afiim", A, F) for A, F in systems]
     if args.no_graphs:
         for h in handles:
@@ -339,7 +339,7 @@ def bench_serve(args, w):
         return results, time.perf_counter() - start
 
     # served: continuous batching, mixed precision
-    srv = cmamd.Server(slots=args.c5_slots, precision=32, small_n=args.c5_small_n)
+    srv = cmamd.Server(slots=args.serve_slots, precision=32, small_n=args.serve_small_n)
     sid = [srv.add_system(h) for h in handles]
     t0 = time.perf_counter()
     for kw in (1, 2, 4, 8, 16):  # create every system's block workspaces once (not timed)
@@ -387,7 +387,7 @@ def bench_serve(args, w):
                 info.append(inf)
             te = time.perf_counter()
             out[i] = (X if i % SAMPLE == 0 else None, info, 1e3 * (ts - t_arr), 1e3 * (te - t_arr))
-    nb = len(reqs) if args.c5_fp64_requests <= 0 else min(len(reqs), args.c5_fp64_requests)
+    nb = len(reqs) if args.serve_fp64_requests <= 0 else min(len(reqs), args.serve_fp64_requests)
     base = [None] * nb
     th = threading.Thread(target=fp64_worker, args=(base,))
     th.start()
@@ -432,11 +432,11 @@ def bench_serve(args, w):
         "warmup": len(systems), "ms_per_step": round(1e3 * t_served / len(reqs), 3), "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "f32 inner / f64 refinement", "data": "synthetic",
         "config": {"workload": w["desc"], "systems": len(systems), "requests": len(reqs),
-                   "rate_per_s": args.c5_rate, "rhs_total": ncols, "slots": args.c5_slots, "tol": w["tol"]},
+                   "rate_per_s": args.serve_rate, "rhs_total": ncols, "slots": args.serve_slots, "tol": w["tol"]},
         "latency_ms": {"p50": _pct(lat_s, 50), "p99": _pct(lat_s, 99)},
         "queue_ms": {"p50": _pct([r[1] for r in served], 50), "p99": _pct([r[1] for r in served], 99)},
         "service_ms": {"p50": _pct([r[7] for r in served], 50), "p99": _pct([r[7] for r in served], 99)},
-        "server": st, "small_n": args.c5_small_n, "warmup_s": round(t_warm, 1),
+        "server": st, "small_n": args.serve_small_n, "warmup_s": round(t_warm, 1),
         "mean_batch_columns": round(float(np.mean([r[2] for r in served])), 2),
         "info_nonzero": int(sum(r[3] for r in served)),
         "fp64_info_nonzero": int(sum(int(np.count_nonzero(b[1])) for b in base)),
@@ -459,25 +459,25 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="c2", choices=sorted(WORKLOADS))
+    ap.add_argument("--workload", default="cd3d-128", choices=sorted(WORKLOADS))
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision legs")
     ap.add_argument("--kernel-timing", choices=("timed-region", "separate"), default="timed-region",
                     help="bracket the dominant kernel's launches with HIP events inside the timed region "
                          "(default), or time the region without events and time that kernel in extra solves")
-    ap.add_argument("--c5-systems", type=int, default=200)
-    ap.add_argument("--c5-requests", type=int, default=10000)
-    ap.add_argument("--c5-rate", type=float, default=200.0, help="Poisson arrival rate (requests/s)")
-    ap.add_argument("--c5-slots", type=int, default=4)
-    ap.add_argument("--c5-seed", type=int, default=5)
-    ap.add_argument("--c5-small-n", type=int, default=8192,
+    ap.add_argument("--serve-systems", type=int, default=200)
+    ap.add_argument("--serve-requests", type=int, default=10000)
+    ap.add_argument("--serve-rate", type=float, default=200.0, help="Poisson arrival rate (requests/s)")
+    ap.add_argument("--serve-slots", type=int, default=4)
+    ap.add_argument("--serve-seed", type=int, default=5)
+    ap.add_argument("--serve-small-n", type=int, default=8192,
                     help="systems with at most this many rows go to the persistent batched kernel (0: never)")
     ap.add_argument("--no-autotune", action="store_true", help="keep the default SpMV row blocks and kernels")
     ap.add_argument("--no-graphs", action="store_true",
-                    help="c4/c5: launch block passes kernel by kernel instead of replaying captured hipGraphs "
+                    help="st27-100-rhs16 / serve-mix: launch block passes kernel by kernel instead of replaying captured hipGraphs "
                          "(for kernel tracers that do not follow graph launches)")
-    ap.add_argument("--c5-fp64-requests", type=int, default=0,
+    ap.add_argument("--serve-fp64-requests", type=int, default=0,
                     help="replay only the first N requests on the FP64 one-at-a-time path (0: all)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="run only a plain device copy of 1 GiB (x10), the reference kernel for scaling the "
@@ -503,9 +503,9 @@ def main():
         print("bench.py: no GPU visible", file=sys.stderr)
         sys.exit(1)
     w = WORKLOADS[args.workload]
-    if args.workload == "c4":
+    if args.workload == "st27-100-rhs16":
         return bench_block(args, w)
-    if args.workload == "c5":
+    if args.workload == "serve-mix":
         return bench_serve(args, w)
     A, xs, b, F, t_build = build_workload(w)
     print(f"[bench] workload {args.workload}: n={A.n} nnz(A)={A.nnz} nnz(G)={F.G.nnz} "

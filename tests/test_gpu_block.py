@@ -143,9 +143,9 @@ def test_block_initial_guess_zero_rhs_and_nmax():
 
 
 @pytest.mark.parametrize("bits", [32, 16])
-@pytest.mark.parametrize("case", ["c1", "c3"])
+@pytest.mark.parametrize("case", ["cd2d_afiim", "cd3d_pe5_acifim"])
 def test_block_mixed_precision(bits, case):
-    if case == "c1":
+    if case == "cd2d_afiim":
         A = cmamd.gen_convdiff(2, 100, 0.5, seed=1)
         F = cmamd.afiim(A, 0.1)
         variant, tol = "afiim", 1e-8

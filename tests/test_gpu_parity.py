@@ -32,7 +32,7 @@ def _trace_close(tr, tro, rel=TRACE_TOL, floor=1e-12):
 
 
 def _system(kind):
-    if kind == "c1":
+    if kind == "cd2d":
         return cmamd.gen_convdiff(2, 100, 0.5, seed=1)
     if kind == "cd3":
         return cmamd.gen_convdiff(3, 20, 1.0, seed=5)
@@ -59,7 +59,7 @@ def _random_csr(n, seed, long_rows=(), empty_rows=()):
     return cmamd.Csr.from_scipy(m)
 
 
-@pytest.mark.parametrize("kind", ["c1", "cd3", "irregular"])
+@pytest.mark.parametrize("kind", ["cd2d", "cd3", "irregular"])
 def test_spmv_matches_oracle(kind):
     A = _system(kind)
     F = cmamd.afiim(A, 0.1)
@@ -85,7 +85,7 @@ def test_spmv_long_and_empty_rows():
     sys_.close()
 
 
-@pytest.mark.parametrize("kind", ["c1", "cd3"])
+@pytest.mark.parametrize("kind", ["cd2d", "cd3"])
 def test_apply_matches_oracle(kind):
     A = _system(kind)
     F = cmamd.afiim(A, 0.1)
@@ -113,7 +113,7 @@ def _compare_solve(variant, A, F, b, nmax, tol=1e-8, **kw):
 
 
 def test_solve_c1_afiim():
-    A = _system("c1")
+    A = _system("cd2d")
     xs = cmamd.gen_uniform(2, A.n)
     b = cmamd.csr_mv_host(A, xs)
     F = cmamd.afiim(A, 0.1)
@@ -146,7 +146,7 @@ def test_solve_3d_acifim(mode, nr):
 
 
 def test_forced_exits():
-    A = _system("c1")
+    A = _system("cd2d")
     xs = cmamd.gen_uniform(2, A.n)
     b = cmamd.csr_mv_host(A, xs)
     F = cmamd.afiim(A, 0.1)
@@ -160,7 +160,7 @@ def test_forced_exits():
 
 
 def test_dropin_entry_points():
-    A = _system("c1")
+    A = _system("cd2d")
     xs = cmamd.gen_uniform(2, A.n)
     b = cmamd.csr_mv_host(A, xs)
     F = cmamd.afiim(A, 0.1)
@@ -205,9 +205,9 @@ def test_pdfiim_symmetric_branch():
 
 
 @pytest.mark.parametrize("bits", [32, 16])
-@pytest.mark.parametrize("case", ["c1", "c3"])
+@pytest.mark.parametrize("case", ["cd2d_afiim", "cd3d_pe5_acifim"])
 def test_mixed_precision_refinement(bits, case):
-    if case == "c1":
+    if case == "cd2d_afiim":
         A = cmamd.gen_convdiff(2, 100, 0.5, seed=1)
         F = cmamd.afiim(A, 0.1)
         variant, tol = "afiim", 1e-8
@@ -267,7 +267,7 @@ def test_graph_replays_match_eager_single():
     """Single-RHS passes replayed from captured hipGraphs against kernel-by-
     kernel launches: bit-identical traces and solutions over several replays,
     with other eager work (a profiled solve, a mixed solve) in between."""
-    A = _system("c1")
+    A = _system("cd2d")
     b = cmamd.csr_mv_host(A, cmamd.gen_uniform(2, A.n))
     F = cmamd.afiim(A, 0.1)
     sys_ = cmamd.System("afiim", A, F)

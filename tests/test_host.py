@@ -23,7 +23,7 @@ REF_SHERMAN = "/root/reference/AFIIM/example/sherman2.mtx"
 @pytest.mark.parametrize("dim,N", [(2, 100), (3, 128), (2, 7), (3, 5)])
 def test_convdiff_sizes(dim, N):
     n, nnz = N ** dim, (2 * dim + 1) * N ** dim - 2 * dim * N ** (dim - 1)
-    if n > 100000:  # C2: only the size query
+    if n > 100000:  # 128^3: only the size query
         import ctypes as C
         nn, zz = C.c_int(), C.c_longlong()
         assert cmamd.host_lib().cm_gen_convdiff(dim, N, 1.0, 2, C.byref(nn), C.byref(zz), None, None, None) == 0

@@ -840,21 +840,28 @@ struct Krylov {
         const long long nmax_passes = std::max(NMAX, 0);
         CM_TRY(push((int)std::min<long long>(4, std::max<long long>(nmax_passes, 1))));
         bool have_prev = false;
+        int prev_iter = -1;      // the snapshot before the latest one
+        double prev_rc = 0.0;
         for (;;) {
             if (nq < 2 && enq < nmax_passes) {
-                long long want = std::min(2 * last_chunk, 16);
+                // Passes still needed, predicted from the faster of the mean
+                // contraction rate since the start and the rate over the last
+                // chunk (a residual that plateaus and then drops fast would
+                // otherwise keep the chunks long past the exit); half of them
+                // are enqueued now, so the chunks shrink towards the exit.
+                // Without a contraction yet, the chunk doubles up to 8.
+                long long want = std::min(2 * last_chunk, 8);
                 if (have_prev) {
-                    // passes still needed at the mean contraction rate since
-                    // the start, of which half are enqueued now: the chunks
-                    // shrink towards the exit, so few no-op passes run past it
                     const SolveState &sn = fin_state;  // latest snapshot
-                    if (sn.iter >= 2 && sn.rc > 0.0 && sn.ro > 0.0 && sn.rc < sn.ro) {
-                        const double rate = std::pow(sn.rc / sn.ro, 1.0 / sn.iter);
-                        if (rate < 1.0 && rate > 0.0) {
-                            const double need = std::log(sn.tol * sn.ro / sn.rc) / std::log(rate);
-                            const long long ahead = enq - sn.iter;
-                            want = (long long)std::max(2.0, std::min(16.0, 0.5 * (need - (double)ahead)));
-                        }
+                    double rate = 2.0;
+                    if (sn.iter >= 2 && sn.rc > 0.0 && sn.ro > 0.0 && sn.rc < sn.ro)
+                        rate = std::pow(sn.rc / sn.ro, 1.0 / sn.iter);
+                    if (prev_iter >= 0 && sn.iter > prev_iter && sn.rc > 0.0 && prev_rc > sn.rc)
+                        rate = std::min(rate, std::pow(sn.rc / prev_rc, 1.0 / (sn.iter - prev_iter)));
+                    if (rate < 1.0 && rate > 0.0) {
+                        const double need = std::log(sn.tol * sn.ro / sn.rc) / std::log(rate);
+                        const long long ahead = enq - sn.iter;
+                        want = (long long)std::max(1.0, std::min(16.0, std::ceil(0.5 * (need - (double)ahead))));
                     }
                 }
                 want = std::min(want, nmax_passes - enq);
@@ -866,6 +873,10 @@ struct Krylov {
             nq--;
             CM_HIP(hipEventSynchronize(ev[o]));
             const SolveState snap = hst[1 + o];
+            if (have_prev) {
+                prev_iter = fin_state.iter;
+                prev_rc = fin_state.rc;
+            }
             fin_state = snap;
             have_prev = true;
             if (snap.done) break;
@@ -1392,7 +1403,7 @@ struct BlockKrylov {
                 if (rate < 1.0 && rate > 0.0) need = std::max(need, std::log(q.tol * q.ro / q.rc) / std::log(rate));
             }
             const long long ahead = enq - hst[KB * (1 + o)].iter;  // passes already queued beyond this snapshot
-            chunk = (int)std::min<double>(16.0, std::max<double>(2.0, 0.5 * (need - (double)std::max<long long>(ahead, 0))));
+            chunk = (int)std::min<double>(16.0, std::max<double>(1.0, std::ceil(0.5 * (need - (double)std::max<long long>(ahead, 0)))));
         }
         CM_HIP(hipStreamSynchronize(stream));
         CM_HIP(hipGetLastError());

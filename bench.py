@@ -352,10 +352,18 @@ def bench_serve(args, w):
     served_raw, t_served = replay(lambda s, B: srv.submit(sid[s], B, tol=w["tol"], nmax=w["nmax"]), srv.wait)
     st = srv.stats()
     served = []
+    info_counts, failed = {}, []
     for i, r in enumerate(served_raw):
         wr, wx = check(i, r.X)
         served.append((r.latency_ms, r.queue_ms, r.batch_cols, int(np.count_nonzero(r.info)), wr, wx,
                        r.X if i % SAMPLE == 0 else None, r.service_ms))
+        for code in r.info:
+            info_counts[int(code)] = info_counts.get(int(code), 0) + 1
+        if np.any(r.info != 0) and len(failed) < 20:
+            s, cols = reqs[i]
+            failed.append({"request": i, "n": int(systems[s][0].n), "k": len(cols), "batch_cols": r.batch_cols,
+                           "info": [int(c) for c in r.info], "resval": [float(v) for v in r.resval],
+                           "outer": r.outer, "inner": r.inner, "true_rel_residual": wr})
     del served_raw
     for key in ("batches", "columns", "persistent_batches", "persistent_columns", "evictions"):
         st[key] -= st0[key]
@@ -438,7 +446,7 @@ def bench_serve(args, w):
         "service_ms": {"p50": _pct([r[7] for r in served], 50), "p99": _pct([r[7] for r in served], 99)},
         "server": st, "small_n": args.serve_small_n, "warmup_s": round(t_warm, 1),
         "mean_batch_columns": round(float(np.mean([r[2] for r in served])), 2),
-        "info_nonzero": int(sum(r[3] for r in served)),
+        "info_nonzero": int(sum(r[3] for r in served)), "info_counts": info_counts, "failed_sample": failed,
         "fp64_info_nonzero": int(sum(int(np.count_nonzero(b[1])) for b in base)),
         "true_rel_residual_max": worst_res, "forward_error_vs_xstar_max": worst_x,
         "rel_error_vs_fp64_solution_max": worst_64, "fp64_comparison_sample": f"every {SAMPLE}th request",

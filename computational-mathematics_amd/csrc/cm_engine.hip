@@ -951,13 +951,16 @@ static int create_inner(KI &ki, K64 &k)
 
 // Inner tolerance of one refinement step: inner_tol, or looser when the
 // residual rn only has to shrink by less than that to reach tol * res0 (half
-// of the remaining factor, at most 0.5).  The last step of a refinement then
-// stops as soon as the outer test can pass, instead of over-solving.
+// of the remaining factor, at most 1e-2).  The last step of a refinement then
+// stops well before a full inner solve.  A loose step that fails to reduce
+// the FP64 residual (the low-precision correction is limited by its own
+// rounding there) is followed by a step at inner_tol instead of ending the
+// refinement.
 static inline double adaptive_inner_tol(double inner_tol, double tol, double res0, double rn)
 {
     if (!(rn > 0.0) || !(res0 > 0.0)) return inner_tol;
     const double need = 0.5 * tol * res0 / rn;
-    return std::max(inner_tol, std::min(need, 0.5));
+    return std::max(inner_tol, std::min(need, 1e-2));
 }
 
 // ------------------------------------------------------------- autotuner
@@ -1548,7 +1551,7 @@ struct BlockSolver : BlockBase {
     {
         bi.ncols = ncols;
         std::vector<double> res0(ncols, -1.0), prev(ncols, -1.0);
-        std::vector<int> act(KB, 0), fin_info(ncols, -1);
+        std::vector<int> act(KB, 0), fin_info(ncols, -1), loose(ncols, 0);
         int steps = 0, inner = 0;
         const int g = b64.vgrid();
         for (int o = 0;; o++) {
@@ -1573,18 +1576,21 @@ struct BlockSolver : BlockBase {
                 if (!std::isfinite(rn)) fin_info[j] = 1;
                 else if (!(rn > tol * res0[j])) fin_info[j] = 0;
                 else if (o >= max_outer) fin_info[j] = 3;
-                else if (prev[j] >= 0.0 && !(rn < prev[j])) fin_info[j] = 2;
+                else if (prev[j] >= 0.0 && !(rn < prev[j]) && !loose[j]) fin_info[j] = 2;
                 else {
+                    act[j] = (prev[j] >= 0.0 && !(rn < prev[j])) ? 2 : 1;  // 2: retry at inner_tol
                     prev[j] = rn;
-                    act[j] = 1;
                     any = true;
                 }
             }
             steps = o;
             if (!any) break;
             std::vector<double> itol(ncols, inner_tol);
-            for (int j = 0; j < ncols; j++)
-                if (act[j]) itol[j] = adaptive_inner_tol(inner_tol, tol, res0[j], host_ost[j].res_outer);
+            for (int j = 0; j < ncols; j++) {
+                if (act[j] == 1) itol[j] = adaptive_inner_tol(inner_tol, tol, res0[j], host_ost[j].res_outer);
+                loose[j] = act[j] && itol[j] > inner_tol;
+                if (act[j]) act[j] = 1;
+            }
             CM_TRY(bi.begin(inner_tol, NMAX, act.data(), itol.data()));
             bi.setup_zero();
             SolveState fin[KB];
@@ -1840,6 +1846,7 @@ static int solve_mixed(System &S, KI &ki, double tol, double inner_tol, int NMAX
     const int g = vec_grid(n);
     out = MixedStats{};
     double res0 = -1.0, prev = -1.0;
+    bool loose = false;  // the last step ran at a tolerance looser than inner_tol
     for (int outer = 0;; outer++) {
         std::memset(S.host_ost, 0, sizeof(SolveState));
         CM_HIP(hipMemcpyAsync(S.ost, S.host_ost, sizeof(SolveState), hipMemcpyHostToDevice, S.stream));
@@ -1862,12 +1869,15 @@ static int solve_mixed(System &S, KI &ki, double tol, double inner_tol, int NMAX
             out.info = 3;
             return CM_OK;
         }
-        if (prev >= 0.0 && !(rn < prev)) {
+        const bool stalled = prev >= 0.0 && !(rn < prev);
+        if (stalled && !loose) {
             out.info = 2;
             return CM_OK;
         }
         prev = rn;
-        CM_TRY(ki.begin(adaptive_inner_tol(inner_tol, tol, res0, rn), NMAX, 0, 0));
+        const double itol = stalled ? inner_tol : adaptive_inner_tol(inner_tol, tol, res0, rn);
+        loose = itol > inner_tol;
+        CM_TRY(ki.begin(itol, NMAX, 0, 0));
         ki.setup_zero();
         SolveState fin;
         CM_TRY(ki.run(NMAX, fin));

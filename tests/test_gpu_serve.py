@@ -130,3 +130,41 @@ def test_admission_by_footprint():
     srv.close()
     h0.close()
     h1.close()
+
+
+def test_served_mix_all_converge():
+    """A burst of requests over a mix of 2D/3D systems of uneven size (up to
+    ~3e5 rows, Peclet up to 5): every served column reaches true relative
+    residual <= 1e-8 with info 0 -- small systems through the persistent
+    kernel, the others through block refinement (this mix once caught a
+    refinement step that stopped early on a non-reducing loose inner
+    tolerance)."""
+    rng = np.random.default_rng(11)
+    systems = []
+    for s in range(16):
+        d = 2 if s % 2 == 0 else 3
+        n = float(np.exp(rng.uniform(np.log(1e3), np.log(3e5))))
+        N = max(4, int(round(n ** (1.0 / d))))
+        A = cmamd.gen_convdiff(d, N, float(rng.uniform(0.0, 5.0)), seed=2000 + s)
+        systems.append((A, cmamd.afiim(A, 0.1)))
+    handles = [cmamd.System("afiim", A, F) for A, F in systems]
+    srv = cmamd.Server(slots=4, precision=32, small_n=8192)
+    ids = [srv.add_system(h) for h in handles]
+    reqs = []
+    for i in range(160):
+        s = int(rng.integers(len(systems)))
+        A = systems[s][0]
+        k = int(rng.integers(1, 9))
+        B = np.stack([cmamd.csr_mv_host(A, cmamd.gen_uniform(7000 + 16 * i + j, A.n)) for j in range(k)], axis=1)
+        reqs.append((s, B, srv.submit(ids[s], B, tol=1e-8, nmax=1000)))
+    results = [(s, B, srv.wait(r)) for s, B, r in reqs]
+    st = srv.stats()
+    srv.close()
+    for h in handles:
+        h.close()
+    assert st["persistent_columns"] > 0 and st["batches"] > 0
+    for s, B, r in results:
+        A = systems[s][0]
+        assert np.all(r.info == 0), (A.n, r.info, r.resval)
+        for j in range(B.shape[1]):
+            assert np.linalg.norm(B[:, j] - cmamd.csr_mv_host(A, r.X[:, j])) / np.linalg.norm(B[:, j]) <= 1e-8

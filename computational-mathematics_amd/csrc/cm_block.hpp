@@ -238,7 +238,7 @@ __global__ __launch_bounds__(BS) void spmm_short(int cnt, const int *__restrict_
 #pragma unroll
             for (int u = 0; u < 4; u++) {
                 cc[u] = col[k + u];
-                vv[u] = load_val<V, Acc>(val, k + u, false);
+                vv[u] = load_val<V, Acc>(val, k + u);
             }
 #pragma unroll
             for (int u = 0; u < 4; u++) {
@@ -251,7 +251,7 @@ __global__ __launch_bounds__(BS) void spmm_short(int cnt, const int *__restrict_
         for (; k < e; k++) {
             T xv[CPL];
             ld_cols<T, CPL>(x + (size_t)col[k] * KB + c0, xv);
-            const Acc vk = load_val<V, Acc>(val, k, false);
+            const Acc vk = load_val<V, Acc>(val, k);
 #pragma unroll
             for (int i = 0; i < CPL; i++) s[i] += vk * (Acc)xv[i];
         }
@@ -304,7 +304,7 @@ __global__ __launch_bounds__(BS) void spmm_long(int cnt, const int *__restrict__
 #pragma unroll
                 for (int u = 0; u < 4; u++) {
                     cc[u] = col[k + u * G];
-                    vv[u] = load_val<V, Acc>(val, k + u * G, false);
+                    vv[u] = load_val<V, Acc>(val, k + u * G);
                 }
 #pragma unroll
                 for (int u = 0; u < 4; u++) {
@@ -317,7 +317,7 @@ __global__ __launch_bounds__(BS) void spmm_long(int cnt, const int *__restrict__
             for (; k < e; k += G) {
                 T xv[CPL];
                 ld_cols<T, CPL>(x + (size_t)col[k] * KB + c0, xv);
-                const Acc vk = load_val<V, Acc>(val, k, false);
+                const Acc vk = load_val<V, Acc>(val, k);
 #pragma unroll
                 for (int i = 0; i < CPL; i++) s[i] += vk * (Acc)xv[i];
             }

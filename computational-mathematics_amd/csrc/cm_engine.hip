@@ -127,6 +127,18 @@ static std::vector<int> make_row_blocks(int n, const int *ptr, int tile = TILE)
     return b;
 }
 
+// The SpMV kernels' row-block table: the block row starts, then the offsets
+// of the blocks' first nonzeros (ptr at those rows); 2 (nblk + 1) entries.
+static std::vector<int> block_table(int n, const int *ptr, int *nblk, int tile = TILE)
+{
+    std::vector<int> b = make_row_blocks(n, ptr, tile);
+    const size_t nb1 = b.size();
+    *nblk = (int)nb1 - 1;
+    b.resize(2 * nb1);
+    for (size_t i = 0; i < nb1; i++) b[nb1 + i] = ptr[b[i]];
+    return b;
+}
+
 template <class T>
 static int dev_alloc(T **d, size_t count)
 {
@@ -178,7 +190,8 @@ struct DevCsr {
     int nblk = 0;
     int tile = TILE;  // staged nonzeros per SpMV workgroup the row blocks were cut for
     bool owns_index = true;
-    int *ptr = nullptr, *col = nullptr, *blk = nullptr;
+    int *ptr = nullptr, *col = nullptr;
+    int *blk = nullptr;  // SpMV row-block table: block row starts, then their first nonzeros
     uint16_t *d16 = nullptr;  // optional block-relative 16-bit columns (col - cbase[block])
     int *cbase = nullptr;
     bool use16 = false;       // low-precision copy: products through spmv_vl16
@@ -194,7 +207,7 @@ struct DevCsr {
         if (!present()) return 0;
         size_t b = (size_t)nnz * sizeof(V) + (rowscale ? (size_t)n * sizeof(float) : 0);
         if (owns_index) {
-            b += (size_t)(n + 1 + nblk + 1) * sizeof(int) + (size_t)nnz * sizeof(int) + (lrows ? (size_t)n * sizeof(int) : 0);
+            b += (size_t)(n + 1 + 2 * (nblk + 1)) * sizeof(int) + (size_t)nnz * sizeof(int) + (lrows ? (size_t)n * sizeof(int) : 0);
             if (d16) b += (size_t)(nnz + 2 * CSR_PAD) * sizeof(uint16_t) + (size_t)nblk * sizeof(int);
         }
         return b;
@@ -238,8 +251,7 @@ static int upload_csr(DevCsr<double> &m, int n, const int *ptr, const int *col, 
     m.owns_index = true;
     m.n = n;
     m.nnz = ptr[n];
-    std::vector<int> blk = make_row_blocks(n, ptr);
-    m.nblk = (int)blk.size() - 1;
+    std::vector<int> blk = block_table(n, ptr, &m.nblk);
     CM_TRY(dev_upload(&m.ptr, ptr, (size_t)n + 1, s));
     CM_TRY(dev_upload_padded(&m.col, col, (size_t)m.nnz, s));
     CM_TRY(dev_upload_padded(&m.val, val, (size_t)m.nnz, s));
@@ -621,7 +633,7 @@ struct Krylov {
     double spmv_bytes(const DevCsr<V> &m) const
     {
         const bool i16 = m.d16 && m.use16 && !std::is_same<V, double>::value;
-        return (double)m.nnz * (sizeof(V) + (i16 ? 2.0 : 4.0)) + 4.0 * (m.n + 1) + 4.0 * (m.nblk + 1) +
+        return (double)m.nnz * (sizeof(V) + (i16 ? 2.0 : 4.0)) + 4.0 * (m.n + 1) + 8.0 * (m.nblk + 1) +
                (i16 ? 4.0 * m.nblk : 0.0) + (m.rowscale ? 4.0 * m.n : 0.0) + (double)sizeof(T) * m.n * (2 + Epi::XV);
     }
 
@@ -973,13 +985,14 @@ static int retile(DevCsr<double> &m, int tile, hipStream_t s)
     std::vector<int> ptr(m.n + 1);
     CM_HIP(hipMemcpyAsync(ptr.data(), m.ptr, (size_t)(m.n + 1) * sizeof(int), hipMemcpyDeviceToHost, s));
     CM_HIP(hipStreamSynchronize(s));
-    std::vector<int> blk = make_row_blocks(m.n, ptr.data(), tile);
+    int nblk = 0;
+    std::vector<int> blk = block_table(m.n, ptr.data(), &nblk, tile);
     int *nb = nullptr;
     CM_TRY(dev_upload(&nb, blk.data(), blk.size(), s));
     CM_HIP(hipStreamSynchronize(s));
     (void)hipFree(m.blk);
     m.blk = nb;
-    m.nblk = (int)blk.size() - 1;
+    m.nblk = nblk;
     m.tile = tile;
     return CM_OK;
 }

@@ -320,92 +320,28 @@ __device__ __forceinline__ float ldv(const V *p, int k) { return (float)p[k]; }
 template <>
 __device__ __forceinline__ float ldv<__half>(const __half *p, int k) { return __half2float(p[k]); }
 
+// Value loads, plain and nontemporal.  Two functions rather than one with a
+// flag: a select between a nontemporal and a plain load of the same address
+// is folded into one load, and the nontemporal hint is lost.
 template <class V, class Acc>
-__device__ __forceinline__ Acc load_val(const V *p, int k, bool nt)
+__device__ __forceinline__ Acc load_val(const V *p, int k)
 {
-    return nt ? (Acc)ldv_nt(p, k) : (Acc)ldv(p, k);
+    return (Acc)ldv(p, k);
 }
 template <>
-__device__ __forceinline__ double load_val<double, double>(const double *p, int k, bool nt)
+__device__ __forceinline__ double load_val<double, double>(const double *p, int k)
 {
-    return nt ? __builtin_nontemporal_load(&p[k]) : p[k];
+    return p[k];
 }
-
-// y = A x (+ epilogue).  A: CSR ptr/col/val (rowscale: optional per-row
-// factor applied to the row sum, for FP16 values stored scaled); blk:
-// row-block starts (gridDim.x + 1), cut for TL staged nonzeros per block.
-// Products and row sums in Acc.
-template <int TL, class V, class X, class Acc, class Epi>
-__global__ __launch_bounds__(BS) void spmv_kernel(const int *__restrict__ ptr,
-                                                  const int *__restrict__ col,
-                                                  const V *__restrict__ val,
-                                                  const float *__restrict__ rowscale,
-                                                  const int *__restrict__ blk,
-                                                  const X *__restrict__ x, Epi epi,
-                                                  SolveState *st, double *partials)
+template <class V, class Acc>
+__device__ __forceinline__ Acc load_val_nt(const V *p, int k)
 {
-    if (st && st->done) return;
-    __shared__ Acc prod[TL];
-    __shared__ double red[(Epi::K > 0 ? Epi::K : 1) * BS / WAVE];
-    double acc[Epi::K > 0 ? Epi::K : 1];
-#pragma unroll
-    for (int k = 0; k < (Epi::K > 0 ? Epi::K : 1); k++) acc[k] = 0.0;
-
-    const int tid = threadIdx.x;
-    const int bi = xcd_block(blockIdx.x, gridDim.x);
-    const int r0 = blk[bi], r1 = blk[bi + 1];
-    const int rs = ptr[r0], re = ptr[r1];
-    const int cnt = re - rs;
-    const int nr = r1 - r0;
-
-    if (cnt > TL) {
-        // one long row: the whole workgroup strides over it
-        double s = 0.0;
-        for (int k = rs + tid; k < re; k += BS) s += (double)(load_val<V, Acc>(val, k, false) * (Acc)x[col[k]]);
-        double v1[1] = {s};
-        block_sum<1>(v1, red);
-        if (tid == 0) {
-            Acc rsum = (Acc)v1[0];
-            if (rowscale) rsum *= (Acc)rowscale[r0];
-            epi.row(r0, rsum, acc);
-        }
-    } else {
-        int cidx[TL / BS];
-        Acc cv[TL / BS];
-#pragma unroll
-        for (int u = 0; u < TL / BS; u++) {
-            const int k = tid + u * BS;
-            if (k < cnt) {  // streamed once: nontemporal, keeps x in the caches
-                cidx[u] = __builtin_nontemporal_load(&col[rs + k]);
-                cv[u] = load_val<V, Acc>(val, rs + k, true);
-            }
-        }
-#pragma unroll
-        for (int u = 0; u < TL / BS; u++) {
-            const int k = tid + u * BS;
-            if (k < cnt) prod[k] = cv[u] * (Acc)x[cidx[u]];
-        }
-        __syncthreads();
-        int tpr = 1;  // lanes per row: a power of two, at most one wave
-        while (tpr < WAVE && tpr * 2 * nr <= BS) tpr *= 2;
-        const int groups = BS / tpr;
-        const int g = tid / tpr, lane = tid & (tpr - 1);
-        for (int base = 0; base < nr; base += groups) {
-            const int rr = base + g;
-            const bool live = rr < nr;
-            Acc s = 0;
-            if (live) {
-                const int a = ptr[r0 + rr] - rs, e = ptr[r0 + rr + 1] - rs;
-                for (int k = a + lane; k < e; k += tpr) s += prod[k];
-            }
-            for (int off = tpr / 2; off > 0; off >>= 1) s += __shfl_xor(s, off, WAVE);
-            if (live && lane == 0) {
-                if (rowscale) s *= (Acc)rowscale[r0 + rr];
-                epi.row(r0 + rr, s, acc);
-            }
-        }
-    }
-    if constexpr (Epi::K > 0) block_partials<Epi::K>(acc, partials, red, bi);
+    return (Acc)ldv_nt(p, k);
+}
+template <>
+__device__ __forceinline__ double load_val_nt<double, double>(const double *p, int k)
+{
+    return __builtin_nontemporal_load(&p[k]);
 }
 
 // Matrix arrays are allocated with this many spare elements after nnz, so
@@ -419,6 +355,142 @@ template <class V>
 __device__ __forceinline__ v4i_t ld16_nt(const V *p)
 {
     return __builtin_nontemporal_load(reinterpret_cast<const v4i_t *>(p));
+}
+
+// Row-block table of a matrix cut for one SpMV launch of nb = gridDim.x
+// workgroups: blk[0..nb] are the blocks' first rows, blk[nb+1..2nb+1] the
+// offsets of their first nonzeros (ptr at those rows).  Reading both halves
+// at once saves the workgroup a dependent load of the row pointers before
+// it can stream its nonzeros.
+struct RowBlock {
+    int r0, r1, rs, re;
+};
+
+__device__ __forceinline__ RowBlock row_block(const int *__restrict__ blk, int bi)
+{
+    const int nb1 = (int)gridDim.x + 1;
+    return {blk[bi], blk[bi + 1], blk[nb1 + bi], blk[nb1 + bi + 1]};
+}
+
+// Rows of a staged block: a power-of-two group of tpr lanes per row (at most
+// one wave), chosen from the row count.  The bounds of each thread's first
+// row are loaded when the groups are formed, before the nonzeros are
+// streamed, so the row reduction does not wait on them.
+struct RowGroups {
+    int tpr, groups, g, lane, a0, e0;
+    __device__ __forceinline__ RowGroups(const int *__restrict__ ptr, const RowBlock &b)
+    {
+        const int nr = b.r1 - b.r0;
+        tpr = 1;
+        while (tpr < WAVE && tpr * 2 * nr <= BS) tpr *= 2;
+        groups = BS / tpr;
+        g = threadIdx.x / tpr;
+        lane = threadIdx.x & (tpr - 1);
+        a0 = e0 = 0;
+        if (g < nr) {
+            a0 = ptr[b.r0 + g] - b.rs;
+            e0 = ptr[b.r0 + g + 1] - b.rs;
+        }
+    }
+};
+
+// Sums each row's staged products (prod[k] for the block's k-th nonzero)
+// with its lane group and xor-shuffles, then hands the row sum (times the
+// row scale, if any) to the epilogue on the group's first lane.
+template <class Acc, class Epi>
+__device__ __forceinline__ void reduce_rows(const RowGroups &G, const RowBlock &b, const Acc *prod,
+                                            const int *__restrict__ ptr, const float *__restrict__ rowscale,
+                                            const Epi &epi, double *acc)
+{
+    const int nr = b.r1 - b.r0;
+    for (int base = 0; base < nr; base += G.groups) {
+        const int rr = base + G.g;
+        const bool live = rr < nr;
+        Acc s = 0;
+        if (live) {
+            int a = G.a0, e = G.e0;
+            if (base > 0) {
+                a = ptr[b.r0 + rr] - b.rs;
+                e = ptr[b.r0 + rr + 1] - b.rs;
+            }
+            for (int k = a + G.lane; k < e; k += G.tpr) s += prod[k];
+        }
+        for (int off = G.tpr / 2; off > 0; off >>= 1) s += __shfl_xor(s, off, WAVE);
+        if (live && G.lane == 0) {
+            if (rowscale) s *= (Acc)rowscale[b.r0 + rr];
+            epi.row(b.r0 + rr, s, acc);
+        }
+    }
+}
+
+// A block holding one row longer than the tile: the whole workgroup strides
+// over it.  colf(k) gives the k-th nonzero's column.
+template <class V, class X, class Acc, class Epi, class Col>
+__device__ __forceinline__ void long_row(const RowBlock &b, const V *__restrict__ val, Col colf,
+                                         const float *__restrict__ rowscale, const X *__restrict__ x,
+                                         const Epi &epi, double *acc, double *red)
+{
+    double s = 0.0;
+    for (int k = b.rs + (int)threadIdx.x; k < b.re; k += BS) s += (double)(load_val<V, Acc>(val, k) * (Acc)x[colf(k)]);
+    double v1[1] = {s};
+    block_sum<1>(v1, red);
+    if (threadIdx.x == 0) {
+        Acc rsum = (Acc)v1[0];
+        if (rowscale) rsum *= (Acc)rowscale[b.r0];
+        epi.row(b.r0, rsum, acc);
+    }
+}
+
+// y = A x (+ epilogue).  A: CSR ptr/col/val (rowscale: optional per-row
+// factor applied to the row sum, for FP16 values stored scaled); blk: the
+// row-block table (row starts, then nonzero starts; 2 (gridDim.x + 1)
+// entries), cut for TL staged nonzeros per block.  Products and row sums in
+// Acc.  The solve-state test comes after the block header loads, so both are
+// in flight together.
+template <int TL, class V, class X, class Acc, class Epi>
+__global__ __launch_bounds__(BS) void spmv_kernel(const int *__restrict__ ptr,
+                                                  const int *__restrict__ col,
+                                                  const V *__restrict__ val,
+                                                  const float *__restrict__ rowscale,
+                                                  const int *__restrict__ blk,
+                                                  const X *__restrict__ x, Epi epi,
+                                                  SolveState *st, double *partials)
+{
+    __shared__ Acc prod[TL];
+    __shared__ double red[(Epi::K > 0 ? Epi::K : 1) * BS / WAVE];
+    double acc[Epi::K > 0 ? Epi::K : 1];
+#pragma unroll
+    for (int k = 0; k < (Epi::K > 0 ? Epi::K : 1); k++) acc[k] = 0.0;
+
+    const int tid = threadIdx.x;
+    const int bi = xcd_block(blockIdx.x, gridDim.x);
+    const RowBlock b = row_block(blk, bi);
+    if (st && st->done) return;
+    const int cnt = b.re - b.rs;
+
+    if (cnt > TL) {
+        long_row<V, X, Acc>(b, val, [&](int k) { return col[k]; }, rowscale, x, epi, acc, red);
+    } else {
+        const RowGroups G(ptr, b);
+        int cidx[TL / BS];
+        Acc cv[TL / BS];
+#pragma unroll
+        for (int u = 0; u < TL / BS; u++) {
+            const int k = tid + u * BS;
+            if (k < cnt) {  // streamed once: nontemporal, keeps x in the caches
+                cidx[u] = __builtin_nontemporal_load(&col[b.rs + k]);
+                cv[u] = load_val_nt<V, Acc>(val, b.rs + k);
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < TL / BS; u++) {
+            const int k = tid + u * BS;
+            if (k < cnt) prod[k] = cv[u] * (Acc)x[cidx[u]];
+        }
+        __syncthreads();
+        reduce_rows(G, b, prod, ptr, rowscale, epi, acc);
+    }
+    if constexpr (Epi::K > 0) block_partials<Epi::K>(acc, partials, red, bi);
 }
 
 // y = A x (+ epilogue), same contract and row blocks as spmv_kernel, with the
@@ -435,7 +507,6 @@ __global__ __launch_bounds__(BS) void spmv_vl(const int *__restrict__ ptr, const
                                               const int *__restrict__ blk, const X *__restrict__ x, Epi epi,
                                               SolveState *st, double *partials)
 {
-    if (st && st->done) return;
     constexpr int VB = (TL + CSR_PAD) * (int)sizeof(V) > TL * (int)sizeof(Acc) ? (TL + CSR_PAD) * (int)sizeof(V)
                                                                                 : TL * (int)sizeof(Acc);
     __shared__ int scol[TL + CSR_PAD];
@@ -449,23 +520,15 @@ __global__ __launch_bounds__(BS) void spmv_vl(const int *__restrict__ ptr, const
 
     const int tid = threadIdx.x;
     const int bi = xcd_block(blockIdx.x, gridDim.x);
-    const int r0 = blk[bi], r1 = blk[bi + 1];
-    const int rs = ptr[r0], re = ptr[r1];
-    const int cnt = re - rs;
-    const int nr = r1 - r0;
+    const RowBlock b = row_block(blk, bi);
+    if (st && st->done) return;
+    const int cnt = b.re - b.rs;
 
     if (cnt > TL) {
-        double s = 0.0;
-        for (int k = rs + tid; k < re; k += BS) s += (double)(load_val<V, Acc>(val, k, false) * (Acc)x[col[k]]);
-        double v1[1] = {s};
-        block_sum<1>(v1, red);
-        if (tid == 0) {
-            Acc rsum = (Acc)v1[0];
-            if (rowscale) rsum *= (Acc)rowscale[r0];
-            epi.row(r0, rsum, acc);
-        }
+        long_row<V, X, Acc>(b, val, [&](int k) { return col[k]; }, rowscale, x, epi, acc, red);
     } else {
-        const int base = rs & ~(CSR_PAD - 1), shift = rs - base;
+        const RowGroups G(ptr, b);
+        const int base = b.rs & ~(CSR_PAD - 1), shift = b.rs - base;
         const int tot = shift + cnt;  // <= TL + CSR_PAD - 1
         constexpr int CW = 16 / (int)sizeof(int), VW = 16 / (int)sizeof(V);
         for (int q = tid; q < (tot + CW - 1) / CW; q += BS)
@@ -479,7 +542,7 @@ __global__ __launch_bounds__(BS) void spmv_vl(const int *__restrict__ ptr, const
             const int k = tid + u * BS;
             if (k < cnt) {
                 const V vv = sval[k + shift];
-                pr[u] = load_val<V, Acc>(&vv, 0, false) * (Acc)x[scol[k + shift]];
+                pr[u] = load_val<V, Acc>(&vv, 0) * (Acc)x[scol[k + shift]];
             }
         }
         __syncthreads();  // values are dead: the products take their place
@@ -489,24 +552,7 @@ __global__ __launch_bounds__(BS) void spmv_vl(const int *__restrict__ ptr, const
             if (k < cnt) prod[k] = pr[u];
         }
         __syncthreads();
-        int tpr = 1;
-        while (tpr < WAVE && tpr * 2 * nr <= BS) tpr *= 2;
-        const int groups = BS / tpr;
-        const int g = tid / tpr, lane = tid & (tpr - 1);
-        for (int b0 = 0; b0 < nr; b0 += groups) {
-            const int rr = b0 + g;
-            const bool live = rr < nr;
-            Acc s = 0;
-            if (live) {
-                const int a = ptr[r0 + rr] - rs, e = ptr[r0 + rr + 1] - rs;
-                for (int k = a + lane; k < e; k += tpr) s += prod[k];
-            }
-            for (int off = tpr / 2; off > 0; off >>= 1) s += __shfl_xor(s, off, WAVE);
-            if (live && lane == 0) {
-                if (rowscale) s *= (Acc)rowscale[r0 + rr];
-                epi.row(r0 + rr, s, acc);
-            }
-        }
+        reduce_rows(G, b, prod, ptr, rowscale, epi, acc);
     }
     if constexpr (Epi::K > 0) block_partials<Epi::K>(acc, partials, red, bi);
 }
@@ -524,7 +570,6 @@ __global__ __launch_bounds__(BS) void spmv_vl16(const int *__restrict__ ptr, con
                                                 const float *__restrict__ rowscale, const int *__restrict__ blk,
                                                 const X *__restrict__ x, Epi epi, SolveState *st, double *partials)
 {
-    if (st && st->done) return;
     constexpr int VB = (TL + CSR_PAD) * (int)sizeof(V) > TL * (int)sizeof(Acc) ? (TL + CSR_PAD) * (int)sizeof(V)
                                                                                 : TL * (int)sizeof(Acc);
     __shared__ __attribute__((aligned(16))) uint16_t sd[TL + 2 * CSR_PAD];
@@ -538,24 +583,16 @@ __global__ __launch_bounds__(BS) void spmv_vl16(const int *__restrict__ ptr, con
 
     const int tid = threadIdx.x;
     const int bi = xcd_block(blockIdx.x, gridDim.x);
-    const int r0 = blk[bi], r1 = blk[bi + 1];
-    const int rs = ptr[r0], re = ptr[r1];
-    const int cnt = re - rs;
-    const int nr = r1 - r0;
+    const RowBlock b = row_block(blk, bi);
     const int cb = cbase[bi];
+    if (st && st->done) return;
+    const int cnt = b.re - b.rs;
 
     if (cnt > TL) {
-        double s = 0.0;
-        for (int k = rs + tid; k < re; k += BS) s += (double)(load_val<V, Acc>(val, k, false) * (Acc)x[cb + d16[k]]);
-        double v1[1] = {s};
-        block_sum<1>(v1, red);
-        if (tid == 0) {
-            Acc rsum = (Acc)v1[0];
-            if (rowscale) rsum *= (Acc)rowscale[r0];
-            epi.row(r0, rsum, acc);
-        }
+        long_row<V, X, Acc>(b, val, [&](int k) { return cb + (int)d16[k]; }, rowscale, x, epi, acc, red);
     } else {
-        const int base = rs & ~(CSR_PAD - 1), shift = rs - base;
+        const RowGroups G(ptr, b);
+        const int base = b.rs & ~(CSR_PAD - 1), shift = b.rs - base;
         const int tot = shift + cnt;  // <= TL + CSR_PAD - 1
         constexpr int DW = 16 / (int)sizeof(uint16_t), VW = 16 / (int)sizeof(V);
         for (int q = tid; q < (tot + DW - 1) / DW; q += BS)
@@ -569,7 +606,7 @@ __global__ __launch_bounds__(BS) void spmv_vl16(const int *__restrict__ ptr, con
             const int k = tid + u * BS;
             if (k < cnt) {
                 const V vv = sval[k + shift];
-                pr[u] = load_val<V, Acc>(&vv, 0, false) * (Acc)x[cb + sd[k + shift]];
+                pr[u] = load_val<V, Acc>(&vv, 0) * (Acc)x[cb + sd[k + shift]];
             }
         }
         __syncthreads();  // values are dead: the products take their place
@@ -579,24 +616,7 @@ __global__ __launch_bounds__(BS) void spmv_vl16(const int *__restrict__ ptr, con
             if (k < cnt) prod[k] = pr[u];
         }
         __syncthreads();
-        int tpr = 1;
-        while (tpr < WAVE && tpr * 2 * nr <= BS) tpr *= 2;
-        const int groups = BS / tpr;
-        const int g = tid / tpr, lane = tid & (tpr - 1);
-        for (int b0 = 0; b0 < nr; b0 += groups) {
-            const int rr = b0 + g;
-            const bool live = rr < nr;
-            Acc s = 0;
-            if (live) {
-                const int a = ptr[r0 + rr] - rs, e = ptr[r0 + rr + 1] - rs;
-                for (int k = a + lane; k < e; k += tpr) s += prod[k];
-            }
-            for (int off = tpr / 2; off > 0; off >>= 1) s += __shfl_xor(s, off, WAVE);
-            if (live && lane == 0) {
-                if (rowscale) s *= (Acc)rowscale[r0 + rr];
-                epi.row(r0 + rr, s, acc);
-            }
-        }
+        reduce_rows(G, b, prod, ptr, rowscale, epi, acc);
     }
     if constexpr (Epi::K > 0) block_partials<Epi::K>(acc, partials, red, bi);
 }

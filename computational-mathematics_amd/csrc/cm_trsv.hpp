@@ -28,7 +28,7 @@ __global__ __launch_bounds__(BS) void k_trsv_level(int cnt, const int *__restric
     const double sc = rowscale ? (double)rowscale[i] : 1.0;
     for (int k = ptr[i]; k < ptr[i + 1]; k++) {
         const int j = col[k];
-        const double a = load_val<V, double>(val, k, false) * sc;
+        const double a = load_val<V, double>(val, k) * sc;
         if (j == i) d = a;
         else s -= a * (double)x[j];
     }

@@ -68,6 +68,97 @@ CAT_KERNELS = {"A": "cm::spmv_kernel<2048, double, double, double, cm::EpiAlpha|
                "F2": "cm::spmv_kernel<2048, double, double, double, cm::EpiStore<double> >"}
 
 
+# Kernel name fragment -> category of the FP64 solve's per-kernel table.
+PMC_CATEGORIES = (("EpiScale<double", "F1"), ("EpiStore<double", "F2"), ("EpiAlpha<double", "A"),
+                  ("EpiOmega<double", "A"), ("k_s_update<double", "VEC"), ("k_xr_update<double", "VEC"),
+                  ("k_p_update<double", "VEC"))
+
+
+def pmc_summary(fetch_db, write_db, copy_fetch_db, copy_write_db, bench_json, workload):
+    """HBM bytes per dispatch from two rocprofv3 --pmc runs of the bench (FETCH_SIZE, WRITE_SIZE, KiB
+    per dispatch) scaled by what the same counters read for a plain 1 GiB device copy, per kernel of
+    the FP64 solve; next to the compulsory bytes the bench's own table gives.  Writes
+    profiles/traffic_<workload>.json (read by the bench for roofline.traffic) and prints the table."""
+    import sqlite3
+
+    def per_dispatch(db):
+        con = sqlite3.connect(db)
+        rows = con.execute("select dispatch_id, kernel_name, value from counters_collection").fetchall()
+        con.close()
+        return rows
+
+    gib = float(2 ** 30)
+    scale = {}
+    for name, db in (("fetch", copy_fetch_db), ("write", copy_write_db)):
+        v = [val for _, k, val in per_dispatch(db) if "copyBuffer" in k and val * 1024.0 > 0.25 * gib]
+        if not v:
+            raise SystemExit(f"no 1 GiB copy dispatch in {db}")
+        scale[name] = gib / (1024.0 * float(np.median(v)))
+    with open(bench_json) as f:
+        bj = json.loads(f.read().strip().splitlines()[-1])
+    algo = {c: bj["kernels"][c]["bytes_per_launch"] for c in ("A", "F1", "F2") if c in bj.get("kernels", {})}
+    out, rows = {}, []
+    sums = {}
+    for name, db in (("fetch", fetch_db), ("write", write_db)):
+        by_kernel = {}
+        for _, k, val in per_dispatch(db):
+            for frag, cat in PMC_CATEGORIES:
+                if frag in k:
+                    by_kernel.setdefault((frag, cat), []).append(1024.0 * val * scale[name])
+        for key, v in by_kernel.items():
+            v = np.asarray(v)
+            v = v[v > 0.5 * v.max()]  # executed dispatches: launches after a solve's exit read ~0
+            sums.setdefault(key, {})[name] = (float(v.mean()), len(v))
+    for (frag, cat), d in sums.items():
+        if "fetch" not in d or "write" not in d:
+            continue
+        t = d["fetch"][0] + d["write"][0]
+        rows.append((frag.split("<")[0], cat, d["fetch"][1], d["fetch"][0], d["write"][0], t, algo.get(cat)))
+        if cat in algo:
+            out.setdefault(cat, []).append(t)
+    tr = {c: float(np.mean(v)) for c, v in out.items()}
+    tr["_note"] = (f"HBM bytes per launch from rocprofv3 --pmc FETCH_SIZE (x{scale['fetch']:.3f}) + WRITE_SIZE "
+                   f"(x{scale['write']:.3f}), scaled by a 1 GiB device copy")
+    path = os.path.join(ROOT, "profiles", f"traffic_{workload}.json")
+    with open(path, "w") as f:
+        json.dump(tr, f, indent=1)
+    print(f"scale factors: fetch x{scale['fetch']:.3f}, write x{scale['write']:.3f}")
+    print("| kernel | category | dispatches | fetch MB | write MB | traffic MB | compulsory MB | traffic / compulsory |")
+    print("|---|---|---|---|---|---|---|---|")
+    for k, cat, nd, fe, wr, t, a in rows:
+        ratio = f"{t / a:.3f}" if a else "-"
+        am = f"{a / 1e6:.1f}" if a else "-"
+        print(f"| {k} | {cat} | {nd} | {fe / 1e6:.1f} | {wr / 1e6:.1f} | {t / 1e6:.1f} | {am} | {ratio} |")
+    print(f"wrote {path}")
+
+
+def trace_summary(db, thr_us=5.0, top=25):
+    """Per-kernel durations from a rocprofv3 --kernel-trace result database, as a markdown table:
+    launches, share of GPU time, mean duration, and the launches longer than thr_us (no-op launches
+    after a solve's exit read the device state and return at once)."""
+    import re
+    import sqlite3
+    con = sqlite3.connect(db)
+    cols = [r[1] for r in con.execute("pragma table_info(kernels)")]
+    name_col = "kernel_name" if "kernel_name" in cols else "name"
+    rows = con.execute(f"select {name_col}, start, end from kernels").fetchall()
+    con.close()
+    agg = {}
+    for n, t0, t1 in rows:
+        agg.setdefault(n, []).append((t1 - t0) / 1e3)
+    tot = sum(sum(v) for v in agg.values())
+    out = []
+    for n, v in agg.items():
+        ex = [d for d in v if d > thr_us]
+        out.append((sum(v), re.sub(r"\(.*$", "", n), len(v), sum(v) / tot, sum(v) / len(v), len(ex),
+                    sum(ex) / len(ex) if ex else 0.0))
+    out.sort(reverse=True)
+    print("| kernel | launches | share of GPU time | avg all (us) | executed | avg executed (us) |")
+    print("|---|---|---|---|---|---|")
+    for t, n, c, sh, a, ne, ae in out[:top]:
+        print(f"| `{n}` | {c} | {100 * sh:.1f}% | {a:.1f} | {ne} | {ae:.1f} |")
+
+
 def build_workload(w):
     import cmamd
     if w.get("gen") == "stencil27":
@@ -483,14 +574,25 @@ def main():
                     help="systems with at most this many rows go to the persistent batched kernel (0: never)")
     ap.add_argument("--no-autotune", action="store_true", help="keep the default SpMV row blocks and kernels")
     ap.add_argument("--no-graphs", action="store_true",
-                    help="st27-100-rhs16 / serve-mix: launch block passes kernel by kernel instead of replaying captured hipGraphs "
+                    help="launch solve passes kernel by kernel instead of replaying captured hipGraphs "
                          "(for kernel tracers that do not follow graph launches)")
     ap.add_argument("--serve-fp64-requests", type=int, default=0,
                     help="replay only the first N requests on the FP64 one-at-a-time path (0: all)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="run only a plain device copy of 1 GiB (x10), the reference kernel for scaling the "
                          "FETCH_SIZE / WRITE_SIZE hardware counters in rocprofv3 --pmc runs")
+    ap.add_argument("--pmc-summary", nargs=5, metavar=("FETCH_DB", "WRITE_DB", "COPY_FETCH_DB", "COPY_WRITE_DB", "BENCH_JSON"),
+                    help="summarize rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE result databases of this workload's "
+                         "bench and of --calibrate-copy into profiles/traffic_<workload>.json (no GPU use)")
+    ap.add_argument("--trace-summary", metavar="TRACE_DB",
+                    help="print the per-kernel table of a rocprofv3 --kernel-trace result database (no GPU use)")
     args = ap.parse_args()
+    if args.trace_summary:
+        trace_summary(args.trace_summary)
+        return
+    if args.pmc_summary:
+        pmc_summary(*args.pmc_summary, args.workload)
+        return
     if args.calibrate_copy:
         import torch
         src = torch.ones(2 ** 27, dtype=torch.float64, device="cuda")  # 1 GiB
@@ -520,6 +622,8 @@ def main():
           f"nnz(H)={F.H.nnz} factors built in {t_build:.1f}s", file=sys.stderr, flush=True)
 
     sys_ = cmamd.System(w["prec"], A, F)
+    if args.no_graphs:
+        sys_.graphs(False)
     tuned = None
     if not args.no_autotune:
         # per-matrix SpMV row-block size, and for the low-precision copies the

@@ -681,6 +681,7 @@ def main():
         for bits in (32, 16):
             mixed[f"fp{bits}_factors"] = mixed_leg(sys_, bits, w, A, b, x, args)
             xs_mixed[f"fp{bits}_factors"] = sys_.get_x()
+    formats = sys_.spmv_formats() if tuned else {}
     sys_.close()
 
     value = args.steps / elapsed
@@ -699,9 +700,15 @@ def main():
     d_gbs = by / ms / 1e6
     kname = CAT_KERNELS.get(dom, dom)
     if tuned:
-        tile = {"A": tuned["A"], "F1": tuned["GT"] or tuned["H"], "F2": tuned["G"]}.get(dom)
-        if tile:
-            kname = kname.replace("<2048,", f"<{tile},")
+        mat = {"A": "A", "F1": "GT" if tuned["GT"] else "H", "F2": "G"}.get(dom)
+        fmt = formats.get(64, {}).get(mat, "staged") if mat else "staged"
+        if fmt.startswith("slot"):  # slot-major kernel: spmv_slot<V, X, Acc, Epi, 16-bit offsets>
+            kname = (kname.replace("cm::spmv_kernel<2048, ", "cm::spmv_slot<")[:-2] +
+                     (", true>" if fmt == "slot16" else ", false>"))
+        else:
+            tile = {"A": tuned["A"], "F1": tuned["GT"] or tuned["H"], "F2": tuned["G"]}.get(dom)
+            if tile:
+                kname = kname.replace("<2048,", f"<{tile},")
     roofline = {"bound": "hbm", "kernel": kname, "achieved": round(d_gbs, 1),
                 "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(d_gbs / HBM_PEAK_GBS, 4),
                 "traffic": None, "bytes_per_launch": by / n_l, "avg_us": round(d_avg_us, 2),
@@ -724,6 +731,7 @@ def main():
         "iterations": int(np.median(iters)), "info": info, "resval": rv, "true_rel_residual": true_res,
         "forward_error_vs_xstar": fwd_err,
         "mixed_precision": mixed, "autotune_tiles": tuned,
+        "spmv_formats": {str(k): v for k, v in formats.items()} if tuned else None,
         "pcie_inclusive": {"ms_per_solve": round(1e3 * t_dropin, 1), "iterations": dr.iter, "info": dr.info,
                            "what": "one call of the drop-in entry point with host arrays: upload and check A, "
                                    "the factors and b, solve, copy x back (not the bench value)"},

@@ -280,7 +280,8 @@ SOLVE_SYMBOLS = [
     "cm_system_enable_mixed", "cm_system_solve_mixed",
     "cm_system_block_set", "cm_system_block_get", "cm_system_block_solve", "cm_system_block_solve_mixed",
     "cm_system_graphs", "cm_system_block_graphs", "cm_system_block_release",
-    "cm_system_block_time_spmm", "cm_batch_solve", "cm_system_autotune", "cm_system_block_autotune", "cm_last_error", "cm_device_count",
+    "cm_system_block_time_spmm", "cm_batch_solve", "cm_system_autotune", "cm_system_block_autotune",
+    "cm_system_spmv_format", "cm_system_spmv_formats", "cm_last_error", "cm_device_count",
     "cm_server_create", "cm_server_configure", "cm_server_destroy", "cm_server_add_system", "cm_server_submit", "cm_server_wait",
     "cm_server_stats",
 ]
@@ -331,6 +332,10 @@ def _type_solve(lib):
     lib.cm_batch_solve.restype = C.c_int
     lib.cm_system_autotune.argtypes = [vp, C.c_int, PI]
     lib.cm_system_autotune.restype = C.c_int
+    lib.cm_system_spmv_format.argtypes = [vp, C.c_int, C.c_int, C.c_int]
+    lib.cm_system_spmv_format.restype = C.c_int
+    lib.cm_system_spmv_formats.argtypes = [vp, PI]
+    lib.cm_system_spmv_formats.restype = C.c_int
     lib.cm_system_block_autotune.argtypes = [vp, C.c_int, C.c_int, PI]
     lib.cm_system_block_autotune.restype = C.c_int
     for f in ("cm_system_block_set", "cm_system_block_get", "cm_system_block_solve",
@@ -485,6 +490,26 @@ class System:
         t = np.zeros(6, np.int32)
         _check(self._lib.cm_system_autotune(self._h, reps, _ip(t)))
         return dict(zip(("A", "G", "H", "GT", "L", "U"), (int(v) for v in t)))
+
+    SPMV_FORMATS = ("staged", "staged16", "slot", "slot16")
+    MATRICES = ("A", "G", "H", "GT", "L", "U")
+
+    def spmv_format(self, bits, which, fmt):
+        """SpMV format of one resident copy: bits 64/32/16, which in MATRICES,
+        fmt in SPMV_FORMATS (EngineError when not available)."""
+        _check(self._lib.cm_system_spmv_format(self._h, int(bits), self.MATRICES.index(which),
+                                               self.SPMV_FORMATS.index(fmt)))
+
+    def spmv_formats(self):
+        """{bits: {matrix: format}} of the resident copies."""
+        f = np.zeros(18, np.int32)
+        _check(self._lib.cm_system_spmv_formats(self._h, _ip(f)))
+        out = {}
+        for p, bits in enumerate((64, 32, 16)):
+            d = {m: self.SPMV_FORMATS[int(f[6 * p + q])] for q, m in enumerate(self.MATRICES) if f[6 * p + q] >= 0}
+            if d:
+                out[bits] = d
+        return out
 
     def block_autotune(self, bits=64, reps=10):
         """Short-row limit of the current block's SpMM bins for A, G, H."""

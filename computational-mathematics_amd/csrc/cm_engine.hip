@@ -89,6 +89,15 @@ __global__ __launch_bounds__(BS) void k_half_rows(int n, const int *__restrict__
     for (int k = ptr[i]; k < ptr[i + 1]; k++) dst[k] = __float2half_rn((float)(src[k] / (double)sc));
 }
 
+// dst[k] = src[idx[k]]: a matrix's values in the slot-major order
+template <class V>
+__global__ __launch_bounds__(BS) void k_gather_vals(long long cnt, const int *__restrict__ idx, const V *__restrict__ src,
+                                                    V *__restrict__ dst)
+{
+    for (long long i = (long long)blockIdx.x * BS + threadIdx.x; i < cnt; i += (long long)gridDim.x * BS)
+        dst[i] = src[idx[i]];
+}
+
 // z = a .* d (mode I of the ACIFIM apply)
 template <class T>
 __global__ __launch_bounds__(BS) void k_vmul(int n, const T *__restrict__ a, const T *__restrict__ d,
@@ -178,6 +187,36 @@ static inline int long_grid(long long cnt)
     return (int)std::max<long long>(1, std::min<long long>(g, 8192));
 }
 
+// Index arrays of a matrix's slot-major copy (spmv_slot), shared by its
+// FP64 and low-precision copies; each copy holds its own values in that
+// order, gathered through src (the CSR position of every slot-major nonzero).
+struct SlotIndex {
+    int n = 0, nb = 0;
+    long long nnz = 0;
+    size_t nsoff = 0;
+    int *hdr = nullptr, *soff = nullptr, *lrow = nullptr, *col = nullptr, *cbase = nullptr, *src = nullptr;
+    uint16_t *d16 = nullptr;  // null when a block's columns span 65536 or more
+    size_t bytes() const
+    {
+        return ((size_t)nb * 5 + nsoff + (size_t)n + 2 * (size_t)nnz) * sizeof(int) +
+               (d16 ? (size_t)nnz * sizeof(uint16_t) + (size_t)nb * sizeof(int) : 0);
+    }
+    ~SlotIndex()
+    {
+        (void)hipFree(hdr);
+        (void)hipFree(soff);
+        (void)hipFree(lrow);
+        (void)hipFree(col);
+        (void)hipFree(cbase);
+        (void)hipFree(src);
+        (void)hipFree(d16);
+    }
+};
+
+// SpMV formats of a resident matrix copy: the staged row blocks with 32-bit
+// or 16-bit column indices, or the slot-major blocks with either.
+enum SpmvFormat { FMT_STAGED = 0, FMT_STAGED16 = 1, FMT_SLOT = 2, FMT_SLOT16 = 3 };
+
 // A CSR matrix in HBM with its SpMV row blocks (and, for triangular factors
 // used by the level-scheduled solves, its level schedule).  A low-precision
 // copy shares ptr/col/blk/levels with the FP64 matrix it was cast from
@@ -197,6 +236,10 @@ struct DevCsr {
     bool use16 = false;       // low-precision copy: products through spmv_vl16
     V *val = nullptr;
     float *rowscale = nullptr;
+    std::shared_ptr<SlotIndex> slots;  // slot-major index (shared by the copies), if built
+    V *sval = nullptr;                 // this copy's values in slot-major order, if built
+    bool use_slot = false;             // products through spmv_slot
+    bool slot16 = false;               // ... with its 16-bit column offsets
     int nlev = 0;
     int *lrows = nullptr;
     std::vector<int> loff;
@@ -206,7 +249,9 @@ struct DevCsr {
     {
         if (!present()) return 0;
         size_t b = (size_t)nnz * sizeof(V) + (rowscale ? (size_t)n * sizeof(float) : 0);
+        if (sval) b += (size_t)nnz * sizeof(V);
         if (owns_index) {
+            if (slots) b += slots->bytes();
             b += (size_t)(n + 1 + 2 * (nblk + 1)) * sizeof(int) + (size_t)nnz * sizeof(int) + (lrows ? (size_t)n * sizeof(int) : 0);
             if (d16) b += (size_t)(nnz + 2 * CSR_PAD) * sizeof(uint16_t) + (size_t)nblk * sizeof(int);
         }
@@ -224,10 +269,13 @@ struct DevCsr {
         }
         (void)hipFree(val);
         (void)hipFree(rowscale);
+        (void)hipFree(sval);
         ptr = col = blk = lrows = cbase = nullptr;
         d16 = nullptr;
-        val = nullptr;
+        val = sval = nullptr;
         rowscale = nullptr;
+        slots.reset();
+        use_slot = slot16 = false;
     }
     ~DevCsr() { release(); }
 };
@@ -272,6 +320,7 @@ static int cast_csr(DevCsr<V> &dst, const DevCsr<double> &src, hipStream_t s)
     dst.tile = src.tile;
     dst.d16 = src.d16;
     dst.cbase = src.cbase;
+    dst.slots = src.slots;  // slot-major values are gathered when the format is chosen
     // the staged 16-bit path pays off from a few nonzeros per row up (the
     // autotuner settles it per matrix by timing)
     dst.use16 = src.d16 && src.nnz >= 4LL * src.n;
@@ -327,6 +376,92 @@ static int build_idx16(DevCsr<double> &m, hipStream_t s)
     CM_TRY(dev_upload(&m.d16, d.data(), d.size(), s));
     CM_TRY(dev_upload(&m.cbase, cb.data(), cb.size(), s));
     CM_HIP(hipStreamSynchronize(s));
+    return CM_OK;
+}
+
+// Slot-major index of a resident FP64 matrix (spmv_slot): blocks of
+// SLOT_ROWS rows, rows ordered by length inside a block (longest first,
+// ties in row order), nonzeros slot by slot.  Left absent when a row is
+// longer than SLOT_MAXLEN.  The FP64 values are gathered into the new order.
+static int build_slots(DevCsr<double> &m, hipStream_t s)
+{
+    if (!m.present() || m.slots) return CM_OK;
+    const int n = m.n;
+    std::vector<int> ptr(n + 1), col(m.nnz);
+    CM_HIP(hipMemcpyAsync(ptr.data(), m.ptr, ptr.size() * sizeof(int), hipMemcpyDeviceToHost, s));
+    if (m.nnz) CM_HIP(hipMemcpyAsync(col.data(), m.col, col.size() * sizeof(int), hipMemcpyDeviceToHost, s));
+    CM_HIP(hipStreamSynchronize(s));
+    for (int i = 0; i < n; i++)
+        if (ptr[i + 1] - ptr[i] > SLOT_MAXLEN) return CM_OK;
+    const int nb = (n + SLOT_ROWS - 1) / SLOT_ROWS;
+    std::vector<int> hdr((size_t)nb * 5), soff, lrow(n), scol(m.nnz), src(m.nnz), cb(nb, 0);
+    std::vector<uint16_t> sd(m.nnz);
+    bool fits16 = true;
+    long long pos = 0;
+    std::vector<int> ord(SLOT_ROWS);
+    for (int b = 0; b < nb; b++) {
+        const int r0 = b * SLOT_ROWS, nr = std::min(SLOT_ROWS, n - r0);
+        auto len = [&](int r) { return ptr[r + 1] - ptr[r]; };
+        for (int i = 0; i < nr; i++) ord[i] = i;
+        std::stable_sort(ord.begin(), ord.begin() + nr, [&](int a, int c) { return len(r0 + a) > len(r0 + c); });
+        const int L = nr ? len(r0 + ord[0]) : 0;
+        int lo = INT32_MAX, hi = -1;
+        for (long long k = ptr[r0]; k < ptr[r0 + nr]; k++) {
+            lo = std::min(lo, col[k]);
+            hi = std::max(hi, col[k]);
+        }
+        if (hi >= 0 && hi - lo > 65535) fits16 = false;
+        cb[b] = hi >= 0 ? lo : 0;
+        hdr[5 * (size_t)b + 0] = r0;
+        hdr[5 * (size_t)b + 1] = nr;
+        hdr[5 * (size_t)b + 2] = (int)pos;
+        hdr[5 * (size_t)b + 3] = (int)soff.size();
+        hdr[5 * (size_t)b + 4] = L;
+        int off = 0;
+        for (int j = 0; j < L; j++) {
+            soff.push_back(off);
+            for (int i = 0; i < nr; i++) {
+                const int r = r0 + ord[i];
+                if (len(r) <= j) break;  // rows are in decreasing length
+                const long long k = ptr[r] + j;
+                scol[pos] = col[k];
+                src[pos] = (int)k;
+                sd[pos] = (uint16_t)(col[k] - cb[b]);
+                pos++;
+                off++;
+            }
+        }
+        soff.push_back(off);
+        for (int i = 0; i < nr; i++) lrow[r0 + i] = (len(r0 + ord[i]) << 16) | ord[i];
+    }
+    auto S = std::make_shared<SlotIndex>();
+    S->n = n;
+    S->nb = nb;
+    S->nnz = m.nnz;
+    S->nsoff = soff.size();
+    CM_TRY(dev_upload(&S->hdr, hdr.data(), hdr.size(), s));
+    CM_TRY(dev_upload(&S->soff, soff.data(), soff.size(), s));
+    CM_TRY(dev_upload(&S->lrow, lrow.data(), lrow.size(), s));
+    CM_TRY(dev_upload(&S->col, scol.data(), scol.size(), s));
+    CM_TRY(dev_upload(&S->src, src.data(), src.size(), s));
+    if (fits16) {
+        CM_TRY(dev_upload(&S->d16, sd.data(), sd.size(), s));
+        CM_TRY(dev_upload(&S->cbase, cb.data(), cb.size(), s));
+    }
+    CM_HIP(hipStreamSynchronize(s));  // host temporaries
+    m.slots = S;
+    return CM_OK;
+}
+
+// This copy's values in the slot-major order of its shared index.
+template <class V>
+static int build_slot_vals(DevCsr<V> &m, hipStream_t s)
+{
+    if (!m.present() || !m.slots || m.sval) return CM_OK;
+    CM_TRY(dev_alloc(&m.sval, (size_t)m.nnz));
+    if (m.nnz)
+        k_gather_vals<V><<<long_grid(m.nnz), BS, 0, s>>>(m.nnz, m.slots->src, m.val, m.sval);
+    CM_HIP(hipGetLastError());
     return CM_OK;
 }
 
@@ -632,6 +767,12 @@ struct Krylov {
     template <class V, class Epi>
     double spmv_bytes(const DevCsr<V> &m) const
     {
+        if (m.use_slot && m.sval) {  // slot-major: block headers, slot tables and row order once
+            const SlotIndex &S = *m.slots;
+            return (double)m.nnz * (sizeof(V) + (m.slot16 ? 2.0 : 4.0)) +
+                   4.0 * (5.0 * S.nb + (double)S.nsoff + S.n) + (m.slot16 ? 4.0 * S.nb : 0.0) +
+                   (m.rowscale ? 4.0 * m.n : 0.0) + (double)sizeof(T) * m.n * (2 + Epi::XV);
+        }
         const bool i16 = m.d16 && m.use16 && !std::is_same<V, double>::value;
         return (double)m.nnz * (sizeof(V) + (i16 ? 2.0 : 4.0)) + 4.0 * (m.n + 1) + 8.0 * (m.nblk + 1) +
                (i16 ? 4.0 * m.nblk : 0.0) + (m.rowscale ? 4.0 * m.n : 0.0) + (double)sizeof(T) * m.n * (2 + Epi::XV);
@@ -645,6 +786,21 @@ struct Krylov {
         // fst: a separate state (refinement residual) instead of the solve's
         SolveState *kst = fst ? fst : st;
         SolveState *gate = use_state ? kst : nullptr;
+        if (m.use_slot && m.sval) {
+            const SlotIndex &S = *m.slots;
+            timed(cat, spmv_bytes<V, Epi>(m), [&] {
+                if (m.slot16)
+                    spmv_slot<V, T, Acc, Epi, true><<<S.nb, BS, 0, stream>>>(
+                        S.hdr, S.soff, S.lrow, S.col, S.d16, S.cbase, m.sval, m.rowscale, xin, epi, gate, partials);
+                else
+                    spmv_slot<V, T, Acc, Epi, false><<<S.nb, BS, 0, stream>>>(
+                        S.hdr, S.soff, S.lrow, S.col, S.d16, S.cbase, m.sval, m.rowscale, xin, epi, gate, partials);
+            });
+            if constexpr (Epi::K > 0)
+                timed(KC_FIN, 8.0 * Epi::K * S.nb,
+                      [&] { k_finalize<Epi::K, Epi><<<1, FBS, 0, stream>>>(S.nb, partials, kst, epi); });
+            return;
+        }
         timed(cat, spmv_bytes<V, Epi>(m), [&] {
             if constexpr (!std::is_same<V, double>::value) {
                 if (m.d16 && m.use16) {  // low precision with 16-bit indices
@@ -998,7 +1154,7 @@ static int retile(DevCsr<double> &m, int tile, hipStream_t s)
 }
 
 // A low-precision copy shares the FP64 matrix's index arrays: follow its
-// row blocks.
+// row blocks and its slot-major index.
 template <class V>
 static void follow_blocks(DevCsr<V> &dst, const DevCsr<double> &src)
 {
@@ -1009,42 +1165,97 @@ static void follow_blocks(DevCsr<V> &dst, const DevCsr<double> &src)
     dst.d16 = src.d16;
     dst.cbase = src.cbase;
     if (!dst.d16) dst.use16 = false;
+    if (dst.slots != src.slots) {
+        (void)hipFree(dst.sval);
+        dst.sval = nullptr;
+        dst.slots = src.slots;
+        dst.use_slot = dst.slot16 = false;
+    }
 }
 
-// Autotuner, low-precision copies: per matrix, the staged 16-bit-index
-// product against the 32-bit one (reps products each), the faster kept.
-template <class KI>
-static int tune_low(KI &ki, int reps, hipEvent_t e0, hipEvent_t e1)
+// Selects the SpMV format of one resident copy (SpmvFormat).  The staged
+// 16-bit form exists for the low-precision copies when the 16-bit offsets
+// were built; the slot-major forms when the shared slot-major index was
+// (FP64 matrices whose rows are at most SLOT_MAXLEN long), the 16-bit one
+// when its blocks' column spans allow.  Slot-major values are gathered on
+// first use.
+template <class V>
+static int set_format(DevCsr<V> &m, int f, hipStream_t s)
 {
-    using VF = typename std::remove_reference<decltype(ki.G)>::type;
+    if (!m.present()) return fail(CM_ERR_ARG, "matrix not resident");
+    const bool low = !std::is_same<V, double>::value;
+    switch (f) {
+    case FMT_STAGED:
+        m.use_slot = false;
+        m.use16 = false;
+        return CM_OK;
+    case FMT_STAGED16:
+        if (!low || !m.d16) return fail(CM_ERR_ARG, "staged 16-bit-index format not available for this copy");
+        m.use_slot = false;
+        m.use16 = true;
+        return CM_OK;
+    case FMT_SLOT:
+    case FMT_SLOT16:
+        if (!m.slots) return fail(CM_ERR_ARG, "slot-major format not available (rows too long or index not built)");
+        if (f == FMT_SLOT16 && !m.slots->d16)
+            return fail(CM_ERR_ARG, "slot-major 16-bit offsets not available (column span)");
+        CM_TRY(build_slot_vals(m, s));
+        m.use_slot = true;
+        m.slot16 = f == FMT_SLOT16;
+        return CM_OK;
+    default:
+        return fail(CM_ERR_ARG, "unknown SpMV format");
+    }
+}
+
+template <class V>
+static int get_format(const DevCsr<V> &m)
+{
+    if (!m.present()) return -1;
+    if (m.use_slot && m.sval) return m.slot16 ? FMT_SLOT16 : FMT_SLOT;
+    return (!std::is_same<V, double>::value && m.d16 && m.use16) ? FMT_STAGED16 : FMT_STAGED;
+}
+
+// Average time of reps products of m on the Krylov's stream (one warm-up).
+template <class KI, class V>
+static int time_product(KI &ki, const DevCsr<V> &m, int reps, hipEvent_t e0, hipEvent_t e1, float *ms)
+{
+    using T = typename std::remove_pointer<decltype(ki.x)>::type;
+    ki.spmv(m, ki.x, EpiStore<T>{ki.t}, false, KC_A);
+    CM_HIP(hipEventRecord(e0, ki.stream));
+    for (int i = 0; i < reps; i++) ki.spmv(m, ki.x, EpiStore<T>{ki.t}, false, KC_A);
+    CM_HIP(hipEventRecord(e1, ki.stream));
+    CM_HIP(hipEventSynchronize(e1));
+    CM_HIP(hipEventElapsedTime(ms, e0, e1));
+    return CM_OK;
+}
+
+// Autotuner, per SpMV matrix of one Krylov (A, G, H, GT): every available
+// format timed (reps products each), the fastest kept.  The staged row
+// blocks are the FP64 matrix's current ones.
+template <class KI>
+static int tune_formats(KI &ki, int reps, hipEvent_t e0, hipEvent_t e1)
+{
     auto one = [&](auto &m) -> int {
-        if (!m.present() || !m.d16) return CM_OK;
+        if (!m.present()) return CM_OK;
+        if (m.slots) CM_TRY(ki.ensure_partials(m.slots->nb));
         float best = 1e30f;
-        bool pick = m.use16;
-        for (bool u : {false, true}) {
-            m.use16 = u;
-            ki.spmv(m, ki.x, EpiStore<float>{ki.t}, false, KC_A);
-            CM_HIP(hipEventRecord(e0, ki.stream));
-            for (int i = 0; i < reps; i++) ki.spmv(m, ki.x, EpiStore<float>{ki.t}, false, KC_A);
-            CM_HIP(hipEventRecord(e1, ki.stream));
-            CM_HIP(hipEventSynchronize(e1));
+        int pick = FMT_STAGED;
+        for (int f = FMT_STAGED; f <= FMT_SLOT16; f++) {
+            if (set_format(m, f, ki.stream) != CM_OK) continue;
             float ms = 0.f;
-            CM_HIP(hipEventElapsedTime(&ms, e0, e1));
+            CM_TRY(time_product(ki, m, reps, e0, e1, &ms));
             if (ms < best) {
                 best = ms;
-                pick = u;
+                pick = f;
             }
         }
-        m.use16 = pick;
-        return CM_OK;
+        return set_format(m, pick, ki.stream);
     };
-    (void)sizeof(VF);
     CM_TRY(one(ki.A));
     CM_TRY(one(ki.G));
     CM_TRY(one(ki.H));
     CM_TRY(one(ki.GT));
-    CM_TRY(one(ki.L));
-    CM_TRY(one(ki.U));
     ki.drop_graphs();
     return CM_OK;
 }
@@ -2036,6 +2247,28 @@ struct cm_system_s {
     System sys;
 };
 
+// The Krylov of one precision (64, 32, 16) and its matrix `which` (0..5: A,
+// G, H, GT, L, U), or null.
+template <class F>
+static int with_matrix(System &S, int bits, int which, F f)
+{
+    if (which < 0 || which > 5) return fail(CM_ERR_ARG, "matrix index must be 0..5 (A, G, H, GT, L, U)");
+    auto pick = [&](auto &ki) -> int {  // the matrices of one Krylov differ in value type
+        switch (which) {
+        case 0: return f(ki, ki.A);
+        case 1: return f(ki, ki.G);
+        case 2: return f(ki, ki.H);
+        case 3: return f(ki, ki.GT);
+        case 4: return f(ki, ki.L);
+        default: return f(ki, ki.U);
+        }
+    };
+    if (bits == 64) return pick(*S.k64);
+    if (bits == 32 && S.k32) return pick(*S.k32);
+    if (bits == 16 && S.k16) return pick(*S.k16);
+    return fail(CM_ERR_ARG, "precision not enabled (64, or 32 / 16 after enabling mixed precision)");
+}
+
 extern "C" {
 
 const char *cm_last_error(void) { return g_last_error.c_str(); }
@@ -2199,21 +2432,19 @@ int cm_system_autotune(cm_system h, int reps, int *tiles)
     CM_HIP(hipEventCreate(&e0));
     CM_HIP(hipEventCreate(&e1));
     int maxblk = 0;
+    // staged row blocks: tile per FP64 matrix
     for (int q = 0; q < 6; q++) {
         DevCsr<double> &m = *ms[q];
         if (tiles) tiles[q] = 0;
         if (!m.present()) continue;
+        CM_TRY(set_format(m, FMT_STAGED, k.stream));
         double best = 1e300;
         int best_t = TILE;
         for (int tile : {1024, 2048, 4096}) {
             CM_TRY(retile(m, tile, k.stream));
-            k.spmv(m, k.x, EpiStore<double>{k.t}, false, KC_A);  // warm
-            CM_HIP(hipEventRecord(e0, k.stream));
-            for (int i = 0; i < reps; i++) k.spmv(m, k.x, EpiStore<double>{k.t}, false, KC_A);
-            CM_HIP(hipEventRecord(e1, k.stream));
-            CM_HIP(hipEventSynchronize(e1));
+            CM_TRY(k.ensure_partials(m.nblk));
             float ms_ = 0.f;
-            CM_HIP(hipEventElapsedTime(&ms_, e0, e1));
+            CM_TRY(time_product(k, m, reps, e0, e1, &ms_));
             if (ms_ < best) {
                 best = ms_;
                 best_t = tile;
@@ -2225,32 +2456,77 @@ int cm_system_autotune(cm_system h, int reps, int *tiles)
         maxblk = std::max(maxblk, m.nblk);
         if (tiles) tiles[q] = best_t;
     }
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
+    // slot-major index of the SpMV matrices (A, G, H, GT), then the format
+    // per matrix and precision
+    for (int q = 0; q < 4; q++) {
+        CM_TRY(build_slots(*ms[q], k.stream));
+        if (ms[q]->slots) maxblk = std::max(maxblk, ms[q]->slots->nb);
+    }
     k.drop_graphs();  // their launches carry the old row blocks
-    if (S.k32) S.k32->drop_graphs();
-    if (S.k16) S.k16->drop_graphs();
     CM_TRY(k.ensure_partials(maxblk));
-    hipEvent_t f0, f1;
-    CM_HIP(hipEventCreate(&f0));
-    CM_HIP(hipEventCreate(&f1));
+    CM_TRY(tune_formats(k, reps, e0, e1));
     if (S.k32) {
         K32 &c = *S.k32;
         follow_blocks(c.A, k.A), follow_blocks(c.G, k.G), follow_blocks(c.H, k.H);
         follow_blocks(c.GT, k.GT), follow_blocks(c.L, k.L), follow_blocks(c.U, k.U);
+        c.drop_graphs();
         CM_TRY(c.ensure_partials(maxblk));
-        CM_TRY(tune_low(c, reps, f0, f1));
+        CM_TRY(tune_formats(c, reps, e0, e1));
     }
     if (S.k16) {
         K16 &c = *S.k16;
         follow_blocks(c.A, k.A), follow_blocks(c.G, k.G), follow_blocks(c.H, k.H);
         follow_blocks(c.GT, k.GT), follow_blocks(c.L, k.L), follow_blocks(c.U, k.U);
+        c.drop_graphs();
         CM_TRY(c.ensure_partials(maxblk));
-        CM_TRY(tune_low(c, reps, f0, f1));
+        CM_TRY(tune_formats(c, reps, e0, e1));
     }
-    (void)hipEventDestroy(f0);
-    (void)hipEventDestroy(f1);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
     CM_HIP(hipGetLastError());
+    return CM_OK;
+}
+
+int cm_system_spmv_format(cm_system h, int bits, int which, int format)
+{
+    if (!h) return fail(CM_ERR_ARG, "null handle");
+    System &S = h->sys;
+    K64 &k = *S.k64;
+    DevCsr<double> *ms[] = {&k.A, &k.G, &k.H, &k.GT, &k.L, &k.U};
+    if (which < 0 || which > 5) return fail(CM_ERR_ARG, "matrix index must be 0..5 (A, G, H, GT, L, U)");
+    if ((format == FMT_SLOT || format == FMT_SLOT16) && which < 4 && ms[which]->present() && !ms[which]->slots) {
+        CM_TRY(build_slots(*ms[which], k.stream));
+        for (int bits : {32, 16})  // the low-precision copies share the new index
+            if ((bits == 32 && S.k32) || (bits == 16 && S.k16))
+                CM_TRY(with_matrix(S, bits, which, [&](auto &, auto &m) -> int {
+                    follow_blocks(m, *ms[which]);
+                    return CM_OK;
+                }));
+    }
+    return with_matrix(S, bits, which, [&](auto &ki, auto &m) -> int {
+        CM_TRY(set_format(m, format, ki.stream));
+        if (m.use_slot) CM_TRY(ki.ensure_partials(m.slots->nb));
+        ki.drop_graphs();
+        CM_HIP(hipStreamSynchronize(ki.stream));
+        return CM_OK;
+    });
+}
+
+int cm_system_spmv_formats(cm_system h, int *formats)
+{
+    if (!h || !formats) return fail(CM_ERR_ARG, "bad argument");
+    System &S = h->sys;
+    for (int p = 0; p < 3; p++)
+        for (int q = 0; q < 6; q++) {
+            formats[6 * p + q] = -1;
+            const int bits = p == 0 ? 64 : p == 1 ? 32 : 16;
+            if (bits == 32 && !S.k32) continue;
+            if (bits == 16 && !S.k16) continue;
+            (void)with_matrix(S, bits, q, [&](auto &, auto &m) -> int {
+                formats[6 * p + q] = get_format(m);
+                return CM_OK;
+            });
+        }
     return CM_OK;
 }
 

@@ -621,6 +621,75 @@ __global__ __launch_bounds__(BS) void spmv_vl16(const int *__restrict__ ptr, con
     if constexpr (Epi::K > 0) block_partials<Epi::K>(acc, partials, red, bi);
 }
 
+// Slot-major row blocks, one row per thread.  Rows are cut into blocks of
+// SLOT_ROWS consecutive rows; inside a block the rows are ordered by length
+// (longest first) and the nonzeros are stored slot by slot: first the first
+// nonzero of every row, then the second of every row that has one, and so on.
+// Thread i of the workgroup owns the block's i-th row in that order; its j-th
+// nonzero sits at data start + soff[j] + i.  Consecutive lanes read
+// consecutive column indices and values, and -- rows of a stencil-like matrix
+// sharing their offset pattern -- gather x at consecutive addresses, which is
+// what bounds the staged kernels on short, regular rows.  Each row is summed
+// in its own nonzero order by one thread (no LDS staging, no shuffles), so
+// the FP64 row sums equal a sequential row loop's bit for bit.
+// Taken for matrices whose rows are at most SLOT_MAXLEN long.
+constexpr int SLOT_ROWS = BS;
+constexpr int SLOT_MAXLEN = 64;
+
+// hdr: per block {first row, rows, data start, slot-table start, slots};
+// lrow: per row position {row length << 16 | block-relative row};
+// column indices as col (I16 = false) or as 16-bit offsets d16 from the
+// block's smallest column cbase (I16 = true).
+template <class V, class X, class Acc, class Epi, bool I16>
+__global__ __launch_bounds__(BS) void spmv_slot(const int *__restrict__ hdr, const int *__restrict__ soff,
+                                                const int *__restrict__ lrow, const int *__restrict__ col,
+                                                const uint16_t *__restrict__ d16, const int *__restrict__ cbase,
+                                                const V *__restrict__ val, const float *__restrict__ rowscale,
+                                                const X *__restrict__ x, Epi epi, SolveState *st, double *partials)
+{
+    __shared__ double red[(Epi::K > 0 ? Epi::K : 1) * BS / WAVE];
+    double acc[Epi::K > 0 ? Epi::K : 1];
+#pragma unroll
+    for (int k = 0; k < (Epi::K > 0 ? Epi::K : 1); k++) acc[k] = 0.0;
+
+    const int bi = xcd_block(blockIdx.x, gridDim.x);
+    const int r0 = hdr[5 * bi], nr = hdr[5 * bi + 1], ds = hdr[5 * bi + 2], ss = hdr[5 * bi + 3];
+    const int cb = I16 ? cbase[bi] : 0;
+    if (st && st->done) return;
+    const int i = threadIdx.x;
+    if (i < nr) {
+        // each product rounded before it is added, as in a plain row loop
+        // (no fused multiply-add): the row sum is the sequential CSR sum
+#pragma clang fp contract(off)
+        const int lr = lrow[r0 + i];
+        const int len = lr >> 16, row = r0 + (lr & 0xffff);
+        Acc s = 0;
+        for (int j = 0; j < len; j += 4) {
+            int k[4];
+            bool on[4];
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                on[q] = j + q < len;
+                k[q] = on[q] ? ds + soff[ss + j + q] + i : 0;
+            }
+            Acc v[4];
+            int c[4];
+#pragma unroll
+            for (int q = 0; q < 4; q++)
+                if (on[q]) {
+                    v[q] = load_val_nt<V, Acc>(val, k[q]);
+                    c[q] = I16 ? cb + (int)__builtin_nontemporal_load(&d16[k[q]]) : __builtin_nontemporal_load(&col[k[q]]);
+                }
+#pragma unroll
+            for (int q = 0; q < 4; q++)
+                if (on[q]) s += v[q] * (Acc)x[c[q]];
+        }
+        if (rowscale) s *= (Acc)rowscale[row];
+        epi.row(row, s, acc);
+    }
+    if constexpr (Epi::K > 0) block_partials<Epi::K>(acc, partials, red, bi);
+}
+
 // ----------------------------------------------------------- vector kernels
 
 // s = r - alpha v   (AFIIM/src/bicgstab.cpp:183-185)

@@ -153,10 +153,26 @@ int cm_system_block_time_spmm(cm_system sys, char which, int bits, int reps, dou
 /* Autotuner of the single-RHS SpMV: for each resident matrix (A, G, H,
  * G^T, L, U), times reps FP64 products with row blocks cut for 1024, 2048
  * and 4096 staged nonzeros per workgroup and keeps the fastest (the FP32 /
- * FP16 copies follow).  tiles[6]: the choices, 0 for an absent matrix.
- * Results of later solves are unchanged up to the summation order of the
- * row blocks' partial dot products. */
+ * FP16 copies follow).  Then, for A, G, H and G^T in every enabled
+ * precision, times each available SpMV format (see cm_system_spmv_format)
+ * and keeps the fastest.  tiles[6]: the tile choices, 0 for an absent
+ * matrix.  Results of later solves are unchanged up to the summation order
+ * of the row sums and of the row blocks' partial dot products. */
 int cm_system_autotune(cm_system sys, int reps, int *tiles);
+/* SpMV format of one resident matrix copy: bits 64, 32 or 16 (after
+ * cm_system_enable_mixed), which 0..5 = A, G, H, G^T, L, U, format
+ *   0  staged row blocks, 32-bit column indices;
+ *   1  staged row blocks, 16-bit block-relative column offsets (FP32 / FP16
+ *      copies whose blocks' column spans are below 65536);
+ *   2  slot-major blocks of 256 rows, one row per thread (rows of at most
+ *      64 nonzeros), 32-bit column indices;
+ *   3  slot-major blocks with 16-bit column offsets.
+ * CM_ERR_ARG when the format is not available for that copy.  The
+ * slot-major index is built on first use (and shared by the precisions). */
+int cm_system_spmv_format(cm_system sys, int bits, int which, int format);
+/* Current formats: formats[18] = {FP64, FP32, FP16} x {A, G, H, G^T, L, U},
+ * -1 for an absent matrix or a precision not enabled. */
+int cm_system_spmv_formats(cm_system sys, int *formats);
 /* Autotuner of the current block's SpMM row binning (bits 64, 32 or 16):
  * for A, G and H the short-row limit among 16, 32, 64, 128 nonzeros that
  * times fastest; short_max[3] receives the choices. */

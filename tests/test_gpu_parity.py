@@ -263,6 +263,86 @@ def test_autotuned_tiles_keep_parity():
     assert np.linalg.norm(b - cmamd.csr_mv_host(A, xm)) / np.linalg.norm(b) <= 1e-8
 
 
+def _slot_case(kind):
+    import scipy.sparse as sp
+    if kind == "afiim":
+        A = _system("cd2d")
+        return "afiim", A, A, cmamd.afiim(A, 0.1), {}, False
+    if kind == "pdfiim_sym":
+        A = cmamd.gen_convdiff(3, 14, 0.0, seed=13)  # Pe = 0: symmetric
+        Up = cmamd.Csr.from_scipy(sp.triu(A.to_scipy()).tocsr())
+        F = cmamd.afiim(A, 0.1)
+        return "pdfiim", A, Up, cmamd.Factors(F.G, F.idv, None), {"a_symmetric": True}, True
+    A = cmamd.gen_convdiff(3, 16, 5.0, seed=8)
+    return "acifim", A, A, cmamd.acifim(A, 0.01, 0.1), {"nr": 2, "mode": "C"}, False
+
+
+@pytest.mark.parametrize("fmt", ["slot", "slot16"])
+@pytest.mark.parametrize("kind", ["afiim", "pdfiim_sym", "acifim"])
+def test_slot_major_format_keeps_parity(kind, fmt):
+    """Every SpMV matrix (A, G, H, G^T) in every precision switched to the
+    slot-major format (one row per thread, rows ordered by length in blocks
+    of 256): the FP64 products, preconditioner apply and residual trace meet
+    the oracle bars above, and the mixed solves reach the tolerance."""
+    prec, A, Aup, F, kw, sym = _slot_case(kind)
+    b = cmamd.csr_mv_host(A, cmamd.gen_uniform(32, A.n))
+    sys_ = cmamd.System(prec, Aup, F, **kw)
+    sys_.enable_mixed(32)
+    sys_.enable_mixed(16)
+    switched = 0
+    for bits in (64, 32, 16):
+        for which in ("A", "G", "H", "GT"):
+            try:
+                sys_.spmv_format(bits, which, fmt)
+                switched += 1
+            except cmamd.EngineError:
+                pass  # absent matrix (H of the symmetric branch, G^T elsewhere)
+    f = sys_.spmv_formats()
+    assert f[64]["A"] == fmt and f[32]["A"] == fmt and f[16]["A"] == fmt and f[64]["G"] == fmt
+    assert switched >= 9
+    x = cmamd.gen_uniform(33, A.n)
+    if sym:
+        assert _rel(sys_.spmv("A", x), oracle.csr_mv(Aup, x, "sym_upper")) <= SPMV_TOL
+    else:
+        assert _rel(sys_.spmv("A", x), oracle.csr_mv(A, x)) <= SPMV_TOL
+    okw = {k: v for k, v in kw.items() if k in ("nr", "mode")}
+    # symmetric branch: M = G D^-1 G^T, so the oracle's apply takes H = G^T
+    Fo = cmamd.Factors(F.G, F.idv, cmamd.Csr.from_scipy(F.G.to_scipy().T.tocsr())) if sym else F
+    assert _rel(sys_.apply(x), oracle.apply(Fo, x, **okw)) <= SPMV_TOL
+    sys_.set_rhs(b)
+    sys_.set_x(None)
+    rv, it, info, tr = sys_.solve(tol=1e-8, nmax=2000, trace_cap=64)
+    xo, rvo, ito, info_o, tro = oracle.bicgstab(prec, Aup, b, F, tol=1e-8, nmax=2000, trace_cap=64,
+                                                **({"issym": True} if sym else okw))
+    m = min(20, len(tr), len(tro))
+    assert _trace_close(tr[:m], tro[:m])
+    assert info == info_o == 0 and abs(it - ito) <= max(1, int(np.ceil(0.02 * ito)))
+    for bits in (32, 16):
+        sys_.set_x(None)
+        rv, outer, inner, info = sys_.solve_mixed(bits, tol=1e-8, inner_tol=1e-4, nmax=2000)
+        xm = sys_.get_x()
+        assert info == 0
+        assert np.linalg.norm(b - cmamd.csr_mv_host(A, xm)) / np.linalg.norm(b) <= 1e-8
+    # back to the staged kernels: the same products
+    for which in ("A", "G"):
+        sys_.spmv_format(64, which, "staged")
+    assert _rel(sys_.spmv("A", x), oracle.csr_mv(Aup, x, "sym_upper") if sym else oracle.csr_mv(A, x)) <= SPMV_TOL
+    sys_.close()
+
+
+def test_slot_major_format_refuses_long_rows():
+    """Rows longer than 64 nonzeros keep the staged format; the autotuner
+    then never offers the slot-major one for that matrix."""
+    A = _system("irregular")
+    F = cmamd.afiim(A, 0.1)
+    sys_ = cmamd.System("afiim", A, F)
+    with pytest.raises(cmamd.EngineError):
+        sys_.spmv_format(64, "A", "slot")
+    sys_.autotune(reps=2)
+    assert sys_.spmv_formats()[64]["A"] in ("staged", "staged16")
+    sys_.close()
+
+
 def test_graph_replays_match_eager_single():
     """Single-RHS passes replayed from captured hipGraphs against kernel-by-
     kernel launches: bit-identical traces and solutions over several replays,

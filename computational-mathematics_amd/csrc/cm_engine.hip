@@ -34,6 +34,7 @@
 #include <memory>
 #include <string>
 #include <type_traits>
+#include <unordered_map>
 #include <vector>
 
 #include "cm_kernels.hpp"
@@ -89,13 +90,16 @@ __global__ __launch_bounds__(BS) void k_half_rows(int n, const int *__restrict__
     for (int k = ptr[i]; k < ptr[i + 1]; k++) dst[k] = __float2half_rn((float)(src[k] / (double)sc));
 }
 
-// dst[k] = src[idx[k]]: a matrix's values in the slot-major order
+// dst[k] = src[idx[k]] (0 where idx[k] < 0): a matrix's values in the
+// slot-major or diagonal order
 template <class V>
 __global__ __launch_bounds__(BS) void k_gather_vals(long long cnt, const int *__restrict__ idx, const V *__restrict__ src,
                                                     V *__restrict__ dst)
 {
-    for (long long i = (long long)blockIdx.x * BS + threadIdx.x; i < cnt; i += (long long)gridDim.x * BS)
-        dst[i] = src[idx[i]];
+    for (long long i = (long long)blockIdx.x * BS + threadIdx.x; i < cnt; i += (long long)gridDim.x * BS) {
+        const int j = idx[i];
+        dst[i] = j >= 0 ? src[j] : (V)0.0f;
+    }
 }
 
 // z = a .* d (mode I of the ACIFIM apply)
@@ -213,9 +217,33 @@ struct SlotIndex {
     }
 };
 
+// Index arrays of a matrix's diagonal copy (spmv_dia), shared like
+// SlotIndex: the diagonal offsets, for every diagonal element the CSR
+// position of its entry (-1: none, stored as 0), and the CSR remainder of
+// the entries off those diagonals with their CSR positions.
+struct DiaIndex {
+    int n = 0, nd = 0;
+    long long nrem = 0;
+    int *off = nullptr, *dsrc = nullptr, *rptr = nullptr, *rcol = nullptr, *rsrc = nullptr;
+    size_t bytes() const
+    {
+        return ((size_t)nd + (size_t)nd * n + (rptr ? (size_t)n + 1 : 0) + 2 * (size_t)nrem) * sizeof(int);
+    }
+    ~DiaIndex()
+    {
+        (void)hipFree(off);
+        (void)hipFree(dsrc);
+        (void)hipFree(rptr);
+        (void)hipFree(rcol);
+        (void)hipFree(rsrc);
+    }
+};
+
 // SpMV formats of a resident matrix copy: the staged row blocks with 32-bit
-// or 16-bit column indices, or the slot-major blocks with either.
-enum SpmvFormat { FMT_STAGED = 0, FMT_STAGED16 = 1, FMT_SLOT = 2, FMT_SLOT16 = 3 };
+// or 16-bit column indices, the slot-major blocks with either, or the
+// diagonals with a CSR remainder.
+enum SpmvFormat { FMT_STAGED = 0, FMT_STAGED16 = 1, FMT_SLOT = 2, FMT_SLOT16 = 3, FMT_DIA = 4 };
+constexpr double DIA_FILL = 0.6;  // a diagonal is stored densely when at least this share of rows has it
 
 // A CSR matrix in HBM with its SpMV row blocks (and, for triangular factors
 // used by the level-scheduled solves, its level schedule).  A low-precision
@@ -240,6 +268,9 @@ struct DevCsr {
     V *sval = nullptr;                 // this copy's values in slot-major order, if built
     bool use_slot = false;             // products through spmv_slot
     bool slot16 = false;               // ... with its 16-bit column offsets
+    std::shared_ptr<DiaIndex> dia;     // diagonal index (shared by the copies), if built
+    V *dval = nullptr, *rval = nullptr;  // this copy's diagonal and remainder values, if built
+    bool use_dia = false;              // products through spmv_dia
     int nlev = 0;
     int *lrows = nullptr;
     std::vector<int> loff;
@@ -250,8 +281,10 @@ struct DevCsr {
         if (!present()) return 0;
         size_t b = (size_t)nnz * sizeof(V) + (rowscale ? (size_t)n * sizeof(float) : 0);
         if (sval) b += (size_t)nnz * sizeof(V);
+        if (dval && dia) b += ((size_t)dia->nd * n + (size_t)dia->nrem) * sizeof(V);
         if (owns_index) {
             if (slots) b += slots->bytes();
+            if (dia) b += dia->bytes();
             b += (size_t)(n + 1 + 2 * (nblk + 1)) * sizeof(int) + (size_t)nnz * sizeof(int) + (lrows ? (size_t)n * sizeof(int) : 0);
             if (d16) b += (size_t)(nnz + 2 * CSR_PAD) * sizeof(uint16_t) + (size_t)nblk * sizeof(int);
         }
@@ -270,12 +303,15 @@ struct DevCsr {
         (void)hipFree(val);
         (void)hipFree(rowscale);
         (void)hipFree(sval);
+        (void)hipFree(dval);
+        (void)hipFree(rval);
         ptr = col = blk = lrows = cbase = nullptr;
         d16 = nullptr;
-        val = sval = nullptr;
+        val = sval = dval = rval = nullptr;
         rowscale = nullptr;
         slots.reset();
-        use_slot = slot16 = false;
+        dia.reset();
+        use_slot = slot16 = use_dia = false;
     }
     ~DevCsr() { release(); }
 };
@@ -320,7 +356,8 @@ static int cast_csr(DevCsr<V> &dst, const DevCsr<double> &src, hipStream_t s)
     dst.tile = src.tile;
     dst.d16 = src.d16;
     dst.cbase = src.cbase;
-    dst.slots = src.slots;  // slot-major values are gathered when the format is chosen
+    dst.slots = src.slots;  // slot-major and diagonal values are gathered when the format is chosen
+    dst.dia = src.dia;
     // the staged 16-bit path pays off from a few nonzeros per row up (the
     // autotuner settles it per matrix by timing)
     dst.use16 = src.d16 && src.nnz >= 4LL * src.n;
@@ -461,6 +498,79 @@ static int build_slot_vals(DevCsr<V> &m, hipStream_t s)
     CM_TRY(dev_alloc(&m.sval, (size_t)m.nnz));
     if (m.nnz)
         k_gather_vals<V><<<long_grid(m.nnz), BS, 0, s>>>(m.nnz, m.slots->src, m.val, m.sval);
+    CM_HIP(hipGetLastError());
+    return CM_OK;
+}
+
+// Diagonal index of a resident FP64 matrix (spmv_dia): the diagonals held
+// by at least DIA_FILL of the rows (the DIA_MAX most populated, in offset
+// order), the other entries as a CSR remainder.  Left absent when no
+// diagonal qualifies.
+static int build_dia(DevCsr<double> &m, hipStream_t s)
+{
+    if (!m.present() || m.dia) return CM_OK;
+    const int n = m.n;
+    std::vector<int> ptr(n + 1), col(m.nnz);
+    CM_HIP(hipMemcpyAsync(ptr.data(), m.ptr, ptr.size() * sizeof(int), hipMemcpyDeviceToHost, s));
+    if (m.nnz) CM_HIP(hipMemcpyAsync(col.data(), m.col, col.size() * sizeof(int), hipMemcpyDeviceToHost, s));
+    CM_HIP(hipStreamSynchronize(s));
+    std::unordered_map<int, long long> cnt;
+    for (int i = 0; i < n; i++)
+        for (int k = ptr[i]; k < ptr[i + 1]; k++) cnt[col[k] - i]++;
+    std::vector<std::pair<long long, int>> cand;
+    for (const auto &kv : cnt)
+        if ((double)kv.second >= DIA_FILL * n) cand.push_back({kv.second, kv.first});
+    if (cand.empty()) return CM_OK;
+    std::sort(cand.begin(), cand.end(), [](const auto &a, const auto &b) {
+        return a.first != b.first ? a.first > b.first : a.second < b.second;
+    });
+    if ((int)cand.size() > DIA_MAX) cand.resize(DIA_MAX);
+    std::vector<int> off;
+    for (const auto &c : cand) off.push_back(c.second);
+    std::sort(off.begin(), off.end());
+    const int nd = (int)off.size();
+    std::unordered_map<int, int> slot;
+    for (int d = 0; d < nd; d++) slot[off[d]] = d;
+    std::vector<int> dsrc((size_t)nd * n, -1), rptr(n + 1, 0), rcol, rsrc;
+    for (int i = 0; i < n; i++) {
+        for (int k = ptr[i]; k < ptr[i + 1]; k++) {
+            const auto it = slot.find(col[k] - i);
+            if (it != slot.end()) {
+                dsrc[(size_t)it->second * n + i] = k;
+            } else {
+                rcol.push_back(col[k]);
+                rsrc.push_back(k);
+            }
+        }
+        rptr[i + 1] = (int)rcol.size();
+    }
+    auto D = std::make_shared<DiaIndex>();
+    D->n = n;
+    D->nd = nd;
+    D->nrem = (long long)rcol.size();
+    CM_TRY(dev_upload(&D->off, off.data(), off.size(), s));
+    CM_TRY(dev_upload(&D->dsrc, dsrc.data(), dsrc.size(), s));
+    if (D->nrem) {
+        CM_TRY(dev_upload(&D->rptr, rptr.data(), rptr.size(), s));
+        CM_TRY(dev_upload(&D->rcol, rcol.data(), rcol.size(), s));
+        CM_TRY(dev_upload(&D->rsrc, rsrc.data(), rsrc.size(), s));
+    }
+    CM_HIP(hipStreamSynchronize(s));  // host temporaries
+    m.dia = D;
+    return CM_OK;
+}
+
+// This copy's diagonal and remainder values (gathered from its CSR values).
+template <class V>
+static int build_dia_vals(DevCsr<V> &m, hipStream_t s)
+{
+    if (!m.present() || !m.dia || m.dval) return CM_OK;
+    const DiaIndex &D = *m.dia;
+    const long long nde = (long long)D.nd * D.n;
+    CM_TRY(dev_alloc(&m.dval, (size_t)nde));
+    CM_TRY(dev_alloc(&m.rval, (size_t)std::max<long long>(D.nrem, 1)));
+    k_gather_vals<V><<<long_grid(nde), BS, 0, s>>>(nde, D.dsrc, m.val, m.dval);
+    if (D.nrem) k_gather_vals<V><<<long_grid(D.nrem), BS, 0, s>>>(D.nrem, D.rsrc, m.val, m.rval);
     CM_HIP(hipGetLastError());
     return CM_OK;
 }
@@ -767,6 +877,12 @@ struct Krylov {
     template <class V, class Epi>
     double spmv_bytes(const DevCsr<V> &m) const
     {
+        if (m.use_dia && m.dval) {  // diagonals (zeros included), offsets, remainder once
+            const DiaIndex &D = *m.dia;
+            return (double)D.nd * m.n * sizeof(V) + 4.0 * D.nd + (double)D.nrem * (sizeof(V) + 4.0) +
+                   (D.rptr ? 4.0 * (m.n + 1) : 0.0) + (m.rowscale ? 4.0 * m.n : 0.0) +
+                   (double)sizeof(T) * m.n * (2 + Epi::XV);
+        }
         if (m.use_slot && m.sval) {  // slot-major: block headers, slot tables and row order once
             const SlotIndex &S = *m.slots;
             return (double)m.nnz * (sizeof(V) + (m.slot16 ? 2.0 : 4.0)) +
@@ -786,6 +902,18 @@ struct Krylov {
         // fst: a separate state (refinement residual) instead of the solve's
         SolveState *kst = fst ? fst : st;
         SolveState *gate = use_state ? kst : nullptr;
+        if (m.use_dia && m.dval) {
+            const DiaIndex &D = *m.dia;
+            const int nb = (m.n + BS - 1) / BS;
+            timed(cat, spmv_bytes<V, Epi>(m), [&] {
+                spmv_dia<V, T, Acc, Epi, 8><<<nb, BS, 0, stream>>>(m.n, D.nd, D.off, m.dval, D.rptr, D.rcol, m.rval,
+                                                                   m.rowscale, xin, epi, gate, partials);
+            });
+            if constexpr (Epi::K > 0)
+                timed(KC_FIN, 8.0 * Epi::K * nb,
+                      [&] { k_finalize<Epi::K, Epi><<<1, FBS, 0, stream>>>(nb, partials, kst, epi); });
+            return;
+        }
         if (m.use_slot && m.sval) {
             const SlotIndex &S = *m.slots;
             timed(cat, spmv_bytes<V, Epi>(m), [&] {
@@ -1171,6 +1299,13 @@ static void follow_blocks(DevCsr<V> &dst, const DevCsr<double> &src)
         dst.slots = src.slots;
         dst.use_slot = dst.slot16 = false;
     }
+    if (dst.dia != src.dia) {
+        (void)hipFree(dst.dval);
+        (void)hipFree(dst.rval);
+        dst.dval = dst.rval = nullptr;
+        dst.dia = src.dia;
+        dst.use_dia = false;
+    }
 }
 
 // Selects the SpMV format of one resident copy (SpmvFormat).  The staged
@@ -1186,12 +1321,12 @@ static int set_format(DevCsr<V> &m, int f, hipStream_t s)
     const bool low = !std::is_same<V, double>::value;
     switch (f) {
     case FMT_STAGED:
-        m.use_slot = false;
+        m.use_slot = m.use_dia = false;
         m.use16 = false;
         return CM_OK;
     case FMT_STAGED16:
         if (!low || !m.d16) return fail(CM_ERR_ARG, "staged 16-bit-index format not available for this copy");
-        m.use_slot = false;
+        m.use_slot = m.use_dia = false;
         m.use16 = true;
         return CM_OK;
     case FMT_SLOT:
@@ -1200,8 +1335,15 @@ static int set_format(DevCsr<V> &m, int f, hipStream_t s)
         if (f == FMT_SLOT16 && !m.slots->d16)
             return fail(CM_ERR_ARG, "slot-major 16-bit offsets not available (column span)");
         CM_TRY(build_slot_vals(m, s));
+        m.use_dia = false;
         m.use_slot = true;
         m.slot16 = f == FMT_SLOT16;
+        return CM_OK;
+    case FMT_DIA:
+        if (!m.dia) return fail(CM_ERR_ARG, "diagonal format not available (no diagonal held by enough rows)");
+        CM_TRY(build_dia_vals(m, s));
+        m.use_slot = false;
+        m.use_dia = true;
         return CM_OK;
     default:
         return fail(CM_ERR_ARG, "unknown SpMV format");
@@ -1212,6 +1354,7 @@ template <class V>
 static int get_format(const DevCsr<V> &m)
 {
     if (!m.present()) return -1;
+    if (m.use_dia && m.dval) return FMT_DIA;
     if (m.use_slot && m.sval) return m.slot16 ? FMT_SLOT16 : FMT_SLOT;
     return (!std::is_same<V, double>::value && m.d16 && m.use16) ? FMT_STAGED16 : FMT_STAGED;
 }
@@ -1239,9 +1382,10 @@ static int tune_formats(KI &ki, int reps, hipEvent_t e0, hipEvent_t e1)
     auto one = [&](auto &m) -> int {
         if (!m.present()) return CM_OK;
         if (m.slots) CM_TRY(ki.ensure_partials(m.slots->nb));
+        if (m.dia) CM_TRY(ki.ensure_partials((m.n + BS - 1) / BS));
         float best = 1e30f;
         int pick = FMT_STAGED;
-        for (int f = FMT_STAGED; f <= FMT_SLOT16; f++) {
+        for (int f = FMT_STAGED; f <= FMT_DIA; f++) {
             if (set_format(m, f, ki.stream) != CM_OK) continue;
             float ms = 0.f;
             CM_TRY(time_product(ki, m, reps, e0, e1, &ms));
@@ -2461,6 +2605,8 @@ int cm_system_autotune(cm_system h, int reps, int *tiles)
     for (int q = 0; q < 4; q++) {
         CM_TRY(build_slots(*ms[q], k.stream));
         if (ms[q]->slots) maxblk = std::max(maxblk, ms[q]->slots->nb);
+        CM_TRY(build_dia(*ms[q], k.stream));
+        if (ms[q]->dia) maxblk = std::max(maxblk, (ms[q]->n + BS - 1) / BS);
     }
     k.drop_graphs();  // their launches carry the old row blocks
     CM_TRY(k.ensure_partials(maxblk));
@@ -2494,8 +2640,11 @@ int cm_system_spmv_format(cm_system h, int bits, int which, int format)
     K64 &k = *S.k64;
     DevCsr<double> *ms[] = {&k.A, &k.G, &k.H, &k.GT, &k.L, &k.U};
     if (which < 0 || which > 5) return fail(CM_ERR_ARG, "matrix index must be 0..5 (A, G, H, GT, L, U)");
-    if ((format == FMT_SLOT || format == FMT_SLOT16) && which < 4 && ms[which]->present() && !ms[which]->slots) {
-        CM_TRY(build_slots(*ms[which], k.stream));
+    const bool need_slots = (format == FMT_SLOT || format == FMT_SLOT16) && !ms[which]->slots;
+    const bool need_dia = format == FMT_DIA && !ms[which]->dia;
+    if (which < 4 && ms[which]->present() && (need_slots || need_dia)) {
+        if (need_slots) CM_TRY(build_slots(*ms[which], k.stream));
+        if (need_dia) CM_TRY(build_dia(*ms[which], k.stream));
         for (int bits : {32, 16})  // the low-precision copies share the new index
             if ((bits == 32 && S.k32) || (bits == 16 && S.k16))
                 CM_TRY(with_matrix(S, bits, which, [&](auto &, auto &m) -> int {
@@ -2506,6 +2655,7 @@ int cm_system_spmv_format(cm_system h, int bits, int which, int format)
     return with_matrix(S, bits, which, [&](auto &ki, auto &m) -> int {
         CM_TRY(set_format(m, format, ki.stream));
         if (m.use_slot) CM_TRY(ki.ensure_partials(m.slots->nb));
+        if (m.use_dia) CM_TRY(ki.ensure_partials((m.n + BS - 1) / BS));
         ki.drop_graphs();
         CM_HIP(hipStreamSynchronize(ki.stream));
         return CM_OK;

@@ -690,6 +690,66 @@ __global__ __launch_bounds__(BS) void spmv_slot(const int *__restrict__ hdr, con
     if constexpr (Epi::K > 0) block_partials<Epi::K>(acc, partials, red, bi);
 }
 
+// Diagonal blocks plus a CSR remainder, one row per thread.  Matrices from
+// structured grids keep most of their nonzeros on a few diagonals (offsets
+// col - row shared by almost every row); those diagonals are stored as
+// dense n-vectors dval[d * n + i] = A(i, i + off[d]) (zero where the entry is
+// absent or the column falls outside the matrix), and the entries on other
+// diagonals as a short CSR remainder (rptr / rcol / rval; rptr null when
+// empty).  The diagonal part reads no column indices, and consecutive lanes
+// read consecutive values and consecutive x entries.  Diagonals are taken
+// CHUNK at a time with all their loads in flight; each product is rounded
+// before it is added (no fused multiply-add), in diagonal order, then the
+// remainder in column order.
+constexpr int DIA_MAX = 32;
+
+template <class V, class X, class Acc, class Epi, int CHUNK>
+__global__ __launch_bounds__(BS) void spmv_dia(int n, int nd, const int *__restrict__ off, const V *__restrict__ dval,
+                                               const int *__restrict__ rptr, const int *__restrict__ rcol,
+                                               const V *__restrict__ rval, const float *__restrict__ rowscale,
+                                               const X *__restrict__ x, Epi epi, SolveState *st, double *partials)
+{
+    __shared__ double red[(Epi::K > 0 ? Epi::K : 1) * BS / WAVE];
+    double acc[Epi::K > 0 ? Epi::K : 1];
+#pragma unroll
+    for (int k = 0; k < (Epi::K > 0 ? Epi::K : 1); k++) acc[k] = 0.0;
+
+    const int bi = xcd_block(blockIdx.x, gridDim.x);
+    const int i = bi * BS + (int)threadIdx.x;
+    int ra = 0, re = 0;
+    if (rptr && i < n) {
+        ra = rptr[i];
+        re = rptr[i + 1];
+    }
+    if (st && st->done) return;
+    if (i < n) {
+#pragma clang fp contract(off)
+        Acc s = 0;
+        for (int d0 = 0; d0 < nd; d0 += CHUNK) {
+            Acc v[CHUNK];
+            Acc xv[CHUNK];
+#pragma unroll
+            for (int q = 0; q < CHUNK; q++) {
+                const int d = d0 + q;
+                v[q] = 0;
+                xv[q] = 0;
+                if (d < nd) {
+                    const int c = i + off[d];
+                    v[q] = load_val_nt<V, Acc>(dval + (size_t)d * n, i);
+                    if ((unsigned)c < (unsigned)n) xv[q] = (Acc)x[c];
+                }
+            }
+#pragma unroll
+            for (int q = 0; q < CHUNK; q++)
+                if (d0 + q < nd) s += v[q] * xv[q];
+        }
+        for (int k = ra; k < re; k++) s += load_val<V, Acc>(rval, k) * (Acc)x[rcol[k]];
+        if (rowscale) s *= (Acc)rowscale[i];
+        epi.row(i, s, acc);
+    }
+    if constexpr (Epi::K > 0) block_partials<Epi::K>(acc, partials, red, bi);
+}
+
 // ----------------------------------------------------------- vector kernels
 
 // s = r - alpha v   (AFIIM/src/bicgstab.cpp:183-185)

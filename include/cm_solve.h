@@ -166,9 +166,13 @@ int cm_system_autotune(cm_system sys, int reps, int *tiles);
  *      copies whose blocks' column spans are below 65536);
  *   2  slot-major blocks of 256 rows, one row per thread (rows of at most
  *      64 nonzeros), 32-bit column indices;
- *   3  slot-major blocks with 16-bit column offsets.
+ *   3  slot-major blocks with 16-bit column offsets;
+ *   4  diagonals held by at least 60% of the rows stored as dense n-vectors
+ *      (no column indices), the other entries as a CSR remainder, one row
+ *      per thread (matrices of structured grids).
  * CM_ERR_ARG when the format is not available for that copy.  The
- * slot-major index is built on first use (and shared by the precisions). */
+ * slot-major and diagonal indexes are built on first use (and shared by the
+ * precisions). */
 int cm_system_spmv_format(cm_system sys, int bits, int which, int format);
 /* Current formats: formats[18] = {FP64, FP32, FP16} x {A, G, H, G^T, L, U},
  * -1 for an absent matrix or a precision not enabled. */

@@ -705,8 +705,8 @@ def main():
         if fmt.startswith("slot"):  # slot-major kernel: spmv_slot<V, X, Acc, Epi, 16-bit offsets>
             kname = (kname.replace("cm::spmv_kernel<2048, ", "cm::spmv_slot<")[:-2] +
                      (", true>" if fmt == "slot16" else ", false>"))
-        elif fmt == "dia":  # diagonals + CSR remainder: spmv_dia<V, X, Acc, Epi, 8>
-            kname = kname.replace("cm::spmv_kernel<2048, ", "cm::spmv_dia<")[:-2] + ", 8>"
+        elif fmt == "dia":  # diagonals + CSR remainder: spmv_dia<V, X, Acc, Epi, CHUNK 8, 1 row per thread>
+            kname = kname.replace("cm::spmv_kernel<2048, ", "cm::spmv_dia<")[:-2] + ", 8, 1>"
         else:
             tile = {"A": tuned["A"], "F1": tuned["GT"] or tuned["H"], "F2": tuned["G"]}.get(dom)
             if tile:

@@ -222,12 +222,12 @@ struct SlotIndex {
 // position of its entry (-1: none, stored as 0), and the CSR remainder of
 // the entries off those diagonals with their CSR positions.
 struct DiaIndex {
-    int n = 0, nd = 0;
+    int n = 0, ldn = 0, nd = 0;  // ldn: diagonal stride, n rounded up to DIA_ALIGN
     long long nrem = 0;
     int *off = nullptr, *dsrc = nullptr, *rptr = nullptr, *rcol = nullptr, *rsrc = nullptr;
     size_t bytes() const
     {
-        return ((size_t)nd + (size_t)nd * n + (rptr ? (size_t)n + 1 : 0) + 2 * (size_t)nrem) * sizeof(int);
+        return ((size_t)nd + (size_t)nd * ldn + (rptr ? (size_t)n + 1 : 0) + 2 * (size_t)nrem) * sizeof(int);
     }
     ~DiaIndex()
     {
@@ -244,6 +244,11 @@ struct DiaIndex {
 // diagonals with a CSR remainder.
 enum SpmvFormat { FMT_STAGED = 0, FMT_STAGED16 = 1, FMT_SLOT = 2, FMT_SLOT16 = 3, FMT_DIA = 4 };
 constexpr double DIA_FILL = 0.6;  // a diagonal is stored densely when at least this share of rows has it
+
+// Rows per thread of spmv_dia: FP16 values are read two rows at a time (one
+// 4-byte load per diagonal); FP32 and FP64 one row per thread.
+template <class V>
+constexpr int dia_rpt() { return std::is_same<V, __half>::value ? 2 : 1; }
 
 // A CSR matrix in HBM with its SpMV row blocks (and, for triangular factors
 // used by the level-scheduled solves, its level schedule).  A low-precision
@@ -281,7 +286,7 @@ struct DevCsr {
         if (!present()) return 0;
         size_t b = (size_t)nnz * sizeof(V) + (rowscale ? (size_t)n * sizeof(float) : 0);
         if (sval) b += (size_t)nnz * sizeof(V);
-        if (dval && dia) b += ((size_t)dia->nd * n + (size_t)dia->nrem) * sizeof(V);
+        if (dval && dia) b += ((size_t)dia->nd * dia->ldn + (size_t)dia->nrem) * sizeof(V);
         if (owns_index) {
             if (slots) b += slots->bytes();
             if (dia) b += dia->bytes();
@@ -529,14 +534,15 @@ static int build_dia(DevCsr<double> &m, hipStream_t s)
     for (const auto &c : cand) off.push_back(c.second);
     std::sort(off.begin(), off.end());
     const int nd = (int)off.size();
+    const int ldn = (n + DIA_ALIGN - 1) / DIA_ALIGN * DIA_ALIGN;
     std::unordered_map<int, int> slot;
     for (int d = 0; d < nd; d++) slot[off[d]] = d;
-    std::vector<int> dsrc((size_t)nd * n, -1), rptr(n + 1, 0), rcol, rsrc;
+    std::vector<int> dsrc((size_t)nd * ldn, -1), rptr(n + 1, 0), rcol, rsrc;
     for (int i = 0; i < n; i++) {
         for (int k = ptr[i]; k < ptr[i + 1]; k++) {
             const auto it = slot.find(col[k] - i);
             if (it != slot.end()) {
-                dsrc[(size_t)it->second * n + i] = k;
+                dsrc[(size_t)it->second * ldn + i] = k;
             } else {
                 rcol.push_back(col[k]);
                 rsrc.push_back(k);
@@ -546,6 +552,7 @@ static int build_dia(DevCsr<double> &m, hipStream_t s)
     }
     auto D = std::make_shared<DiaIndex>();
     D->n = n;
+    D->ldn = ldn;
     D->nd = nd;
     D->nrem = (long long)rcol.size();
     CM_TRY(dev_upload(&D->off, off.data(), off.size(), s));
@@ -566,7 +573,7 @@ static int build_dia_vals(DevCsr<V> &m, hipStream_t s)
 {
     if (!m.present() || !m.dia || m.dval) return CM_OK;
     const DiaIndex &D = *m.dia;
-    const long long nde = (long long)D.nd * D.n;
+    const long long nde = (long long)D.nd * D.ldn;
     CM_TRY(dev_alloc(&m.dval, (size_t)nde));
     CM_TRY(dev_alloc(&m.rval, (size_t)std::max<long long>(D.nrem, 1)));
     k_gather_vals<V><<<long_grid(nde), BS, 0, s>>>(nde, D.dsrc, m.val, m.dval);
@@ -904,10 +911,12 @@ struct Krylov {
         SolveState *gate = use_state ? kst : nullptr;
         if (m.use_dia && m.dval) {
             const DiaIndex &D = *m.dia;
-            const int nb = (m.n + BS - 1) / BS;
+            constexpr int RPT = dia_rpt<V>();
+            const int nb = (m.n + BS * RPT - 1) / (BS * RPT);
             timed(cat, spmv_bytes<V, Epi>(m), [&] {
-                spmv_dia<V, T, Acc, Epi, 8><<<nb, BS, 0, stream>>>(m.n, D.nd, D.off, m.dval, D.rptr, D.rcol, m.rval,
-                                                                   m.rowscale, xin, epi, gate, partials);
+                spmv_dia<V, T, Acc, Epi, 8, RPT><<<nb, BS, 0, stream>>>(m.n, D.ldn, D.nd, D.off, m.dval, D.rptr,
+                                                                        D.rcol, m.rval, m.rowscale, xin, epi, gate,
+                                                                        partials);
             });
             if constexpr (Epi::K > 0)
                 timed(KC_FIN, 8.0 * Epi::K * nb,
@@ -1382,7 +1391,7 @@ static int tune_formats(KI &ki, int reps, hipEvent_t e0, hipEvent_t e1)
     auto one = [&](auto &m) -> int {
         if (!m.present()) return CM_OK;
         if (m.slots) CM_TRY(ki.ensure_partials(m.slots->nb));
-        if (m.dia) CM_TRY(ki.ensure_partials((m.n + BS - 1) / BS));
+        if (m.dia) CM_TRY(ki.ensure_partials((m.n + BS - 1) / BS));  // at least the launch's workgroups
         float best = 1e30f;
         int pick = FMT_STAGED;
         for (int f = FMT_STAGED; f <= FMT_DIA; f++) {

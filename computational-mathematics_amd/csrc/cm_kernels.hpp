@@ -690,24 +690,50 @@ __global__ __launch_bounds__(BS) void spmv_slot(const int *__restrict__ hdr, con
     if constexpr (Epi::K > 0) block_partials<Epi::K>(acc, partials, red, bi);
 }
 
-// Diagonal blocks plus a CSR remainder, one row per thread.  Matrices from
-// structured grids keep most of their nonzeros on a few diagonals (offsets
-// col - row shared by almost every row); those diagonals are stored as
-// dense n-vectors dval[d * n + i] = A(i, i + off[d]) (zero where the entry is
-// absent or the column falls outside the matrix), and the entries on other
-// diagonals as a short CSR remainder (rptr / rcol / rval; rptr null when
-// empty).  The diagonal part reads no column indices, and consecutive lanes
-// read consecutive values and consecutive x entries.  Diagonals are taken
-// CHUNK at a time with all their loads in flight; each product is rounded
-// before it is added (no fused multiply-add), in diagonal order, then the
-// remainder in column order.
+// Diagonals plus a CSR remainder.  Matrices from structured grids keep most
+// of their nonzeros on a few diagonals (offsets col - row shared by almost
+// every row); those diagonals are stored as dense vectors dval[d * ldn + i] =
+// A(i, i + off[d]) (zero where the entry is absent or the column falls
+// outside the matrix; ldn = n rounded up to DIA_ALIGN, so every diagonal
+// starts 16-byte aligned), and the entries on other diagonals as a short CSR
+// remainder (rptr / rcol / rval; rptr null when empty).  The diagonal part
+// reads no column indices, and consecutive lanes read consecutive values and
+// consecutive x entries.  Each thread takes RPT consecutive rows, so one
+// diagonal's values for them are one vector load (RPT * sizeof(V) bytes);
+// diagonals are taken CHUNK at a time with all their loads in flight.  Each
+// product is rounded before it is added (no fused multiply-add), in diagonal
+// order, then the remainder in column order.
 constexpr int DIA_MAX = 32;
+constexpr int DIA_ALIGN = 8;
 
-template <class V, class X, class Acc, class Epi, int CHUNK>
-__global__ __launch_bounds__(BS) void spmv_dia(int n, int nd, const int *__restrict__ off, const V *__restrict__ dval,
-                                               const int *__restrict__ rptr, const int *__restrict__ rcol,
-                                               const V *__restrict__ rval, const float *__restrict__ rowscale,
-                                               const X *__restrict__ x, Epi epi, SolveState *st, double *partials)
+template <class V, int RPT>
+struct DiaVec {  // RPT values of V loaded as one aligned vector
+    typedef V type __attribute__((ext_vector_type(RPT)));
+};
+template <class V>
+struct DiaVec<V, 1> {
+    typedef V type;
+};
+template <int RPT>
+struct DiaVec<__half, RPT> {
+    typedef unsigned short type __attribute__((ext_vector_type(RPT)));
+};
+template <>
+struct DiaVec<__half, 1> {
+    typedef unsigned short type;
+};
+
+template <class V, class Acc>
+__device__ __forceinline__ Acc dia_val(V v) { return (Acc)v; }
+template <>
+__device__ __forceinline__ float dia_val<__half, float>(__half v) { return __half2float(v); }
+
+template <class V, class X, class Acc, class Epi, int CHUNK, int RPT>
+__global__ __launch_bounds__(BS) void spmv_dia(int n, int ldn, int nd, const int *__restrict__ off,
+                                               const V *__restrict__ dval, const int *__restrict__ rptr,
+                                               const int *__restrict__ rcol, const V *__restrict__ rval,
+                                               const float *__restrict__ rowscale, const X *__restrict__ x, Epi epi,
+                                               SolveState *st, double *partials)
 {
     __shared__ double red[(Epi::K > 0 ? Epi::K : 1) * BS / WAVE];
     double acc[Epi::K > 0 ? Epi::K : 1];
@@ -715,37 +741,56 @@ __global__ __launch_bounds__(BS) void spmv_dia(int n, int nd, const int *__restr
     for (int k = 0; k < (Epi::K > 0 ? Epi::K : 1); k++) acc[k] = 0.0;
 
     const int bi = xcd_block(blockIdx.x, gridDim.x);
-    const int i = bi * BS + (int)threadIdx.x;
-    int ra = 0, re = 0;
-    if (rptr && i < n) {
-        ra = rptr[i];
-        re = rptr[i + 1];
-    }
+    const int i0 = (bi * BS + (int)threadIdx.x) * RPT;
     if (st && st->done) return;
-    if (i < n) {
+    if (i0 < n) {
 #pragma clang fp contract(off)
-        Acc s = 0;
+        Acc s[RPT];
+#pragma unroll
+        for (int r = 0; r < RPT; r++) s[r] = 0;
+        using VT = typename DiaVec<V, RPT>::type;
         for (int d0 = 0; d0 < nd; d0 += CHUNK) {
-            Acc v[CHUNK];
-            Acc xv[CHUNK];
+            VT v[CHUNK];
+            Acc xv[CHUNK][RPT];
 #pragma unroll
             for (int q = 0; q < CHUNK; q++) {
                 const int d = d0 + q;
-                v[q] = 0;
-                xv[q] = 0;
+#pragma unroll
+                for (int r = 0; r < RPT; r++) xv[q][r] = 0;
                 if (d < nd) {
-                    const int c = i + off[d];
-                    v[q] = load_val_nt<V, Acc>(dval + (size_t)d * n, i);
-                    if ((unsigned)c < (unsigned)n) xv[q] = (Acc)x[c];
+                    const int c = i0 + off[d];
+                    v[q] = __builtin_nontemporal_load(reinterpret_cast<const VT *>(dval + (size_t)d * ldn + i0));
+#pragma unroll
+                    for (int r = 0; r < RPT; r++)
+                        if ((unsigned)(c + r) < (unsigned)n) xv[q][r] = (Acc)x[c + r];
                 }
             }
 #pragma unroll
             for (int q = 0; q < CHUNK; q++)
-                if (d0 + q < nd) s += v[q] * xv[q];
+                if (d0 + q < nd) {
+#pragma unroll
+                    for (int r = 0; r < RPT; r++) {
+                        V e;
+                        if constexpr (RPT == 1) {
+                            __builtin_memcpy(&e, &v[q], sizeof(V));
+                        } else {
+                            const auto lane = v[q][r];
+                            __builtin_memcpy(&e, &lane, sizeof(V));
+                        }
+                        s[r] += dia_val<V, Acc>(e) * xv[q][r];
+                    }
+                }
         }
-        for (int k = ra; k < re; k++) s += load_val<V, Acc>(rval, k) * (Acc)x[rcol[k]];
-        if (rowscale) s *= (Acc)rowscale[i];
-        epi.row(i, s, acc);
+#pragma unroll
+        for (int r = 0; r < RPT; r++) {
+            const int i = i0 + r;
+            if (i >= n) break;
+            Acc t = s[r];
+            if (rptr)
+                for (int k = rptr[i]; k < rptr[i + 1]; k++) t += load_val<V, Acc>(rval, k) * (Acc)x[rcol[k]];
+            if (rowscale) t *= (Acc)rowscale[i];
+            epi.row(i, t, acc);
+        }
     }
     if constexpr (Epi::K > 0) block_partials<Epi::K>(acc, partials, red, bi);
 }

@@ -331,6 +331,32 @@ def test_slot_major_format_keeps_parity(kind, fmt):
     sys_.close()
 
 
+def test_dia_format_odd_row_count():
+    """Diagonal format on a matrix with an odd row count (the FP16 copies
+    take two rows per thread; the last thread has one): FP64 products and
+    apply against the oracle, and FP32 / FP16 refinement to the tolerance."""
+    A = cmamd.gen_convdiff(2, 77, 1.0, seed=41)  # n = 5929
+    F = cmamd.afiim(A, 0.1)
+    b = cmamd.csr_mv_host(A, cmamd.gen_uniform(42, A.n))
+    sys_ = cmamd.System("afiim", A, F)
+    sys_.enable_mixed(32)
+    sys_.enable_mixed(16)
+    for bits in (64, 32, 16):
+        for which in ("A", "G"):
+            sys_.spmv_format(bits, which, "dia")
+    x = cmamd.gen_uniform(43, A.n)
+    assert _rel(sys_.spmv("A", x), oracle.csr_mv(A, x)) <= SPMV_TOL
+    assert _rel(sys_.apply(x), oracle.apply(F, x)) <= SPMV_TOL
+    for bits in (32, 16):
+        sys_.set_rhs(b)
+        sys_.set_x(None)
+        rv, outer, inner, info = sys_.solve_mixed(bits, tol=1e-8, inner_tol=1e-4, nmax=2000)
+        xm = sys_.get_x()
+        assert info == 0
+        assert np.linalg.norm(b - cmamd.csr_mv_host(A, xm)) / np.linalg.norm(b) <= 1e-8
+    sys_.close()
+
+
 def test_slot_major_format_refuses_long_rows():
     """Rows longer than 64 nonzeros keep the staged format; the autotuner
     then never offers the slot-major one for that matrix."""

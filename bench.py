@@ -81,10 +81,15 @@ def pmc_summary(fetch_db, write_db, copy_fetch_db, copy_write_db, bench_json, wo
     profiles/traffic_<workload>.json (read by the bench for roofline.traffic) and prints the table."""
     import sqlite3
 
-    def per_dispatch(db):
+    def per_dispatch(db, solves_only=False):
         con = sqlite3.connect(db)
         rows = con.execute("select dispatch_id, kernel_name, value from counters_collection").fetchall()
         con.close()
+        if solves_only:
+            # the autotuner's timing launches come before the first solve: keep the dispatches from
+            # the first BiCGSTAB vector update on
+            first = min((d for d, k, _ in rows if "k_s_update" in k), default=0)
+            rows = [r for r in rows if r[0] >= first]
         return rows
 
     gib = float(2 ** 30)
@@ -101,7 +106,7 @@ def pmc_summary(fetch_db, write_db, copy_fetch_db, copy_write_db, bench_json, wo
     sums = {}
     for name, db in (("fetch", fetch_db), ("write", write_db)):
         by_kernel = {}
-        for _, k, val in per_dispatch(db):
+        for _, k, val in per_dispatch(db, solves_only=True):
             for frag, cat in PMC_CATEGORIES:
                 if frag in k:
                     by_kernel.setdefault((frag, cat), []).append(1024.0 * val * scale[name])
@@ -255,6 +260,10 @@ def bench_block(args, w):
           f"factors built in {t_build:.1f}s", file=sys.stderr, flush=True)
     sys_ = cmamd.System(w["prec"], A, F)
     sys_.enable_mixed(32)
+    tuned = None
+    if not args.no_autotune:
+        # single-RHS SpMV formats (the FP64 one-column-at-a-time baseline runs on them)
+        tuned = sys_.autotune(reps=10)
     if args.no_graphs:
         sys_.block_graphs(False)
     steps = max(1, args.steps)
@@ -306,6 +315,7 @@ def bench_block(args, w):
     us, by = sys_.block_time_spmm("A", 32, reps=20)
     us_g, by_g = sys_.block_time_spmm("G", 32, reps=20)
     us_h, by_h = sys_.block_time_spmm("H", 32, reps=20)
+    formats = sys_.spmv_formats() if tuned else {}
     sys_.close()
     res = [float(np.linalg.norm(B[:, j] - cmamd.csr_mv_host(A, Xmx[:, j])) / np.linalg.norm(B[:, j]))
            for j in range(k)]
@@ -333,6 +343,7 @@ def bench_block(args, w):
                                                      "from the timed solves"},
         "factor_spmm": {"G": {"avg_us": round(us_g, 2), "GBs": round(by_g / us_g / 1e3, 1)},
                         "H": {"avg_us": round(us_h, 2), "GBs": round(by_h / us_h / 1e3, 1)}},
+        "single_rhs_spmv_formats": {str(b_): f for b_, f in formats.items()} if tuned else None,
     }
     if not args.no_cpu:
         out["cpu_baseline"], x_or = cpu_baseline(w, A, B[:, 0], F, args.cpu_seconds)

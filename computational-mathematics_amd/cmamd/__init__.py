@@ -281,7 +281,7 @@ SOLVE_SYMBOLS = [
     "cm_system_block_set", "cm_system_block_get", "cm_system_block_solve", "cm_system_block_solve_mixed",
     "cm_system_graphs", "cm_system_block_graphs", "cm_system_block_release",
     "cm_system_block_time_spmm", "cm_batch_solve", "cm_system_autotune", "cm_system_block_autotune",
-    "cm_system_spmv_format", "cm_system_spmv_formats", "cm_last_error", "cm_device_count",
+    "cm_system_spmv_format", "cm_system_spmv_formats", "cm_system_profile_sampled", "cm_last_error", "cm_device_count",
     "cm_server_create", "cm_server_configure", "cm_server_destroy", "cm_server_add_system", "cm_server_submit", "cm_server_wait",
     "cm_server_stats",
 ]
@@ -355,6 +355,8 @@ def _type_solve(lib):
         getattr(lib, f).restype = C.c_int
     lib.cm_system_profile.argtypes = [vp, C.c_int]
     lib.cm_system_profile.restype = C.c_int
+    lib.cm_system_profile_sampled.argtypes = [vp, C.c_int, C.c_int]
+    lib.cm_system_profile_sampled.restype = C.c_int
     lib.cm_system_kernel_stats.argtypes = [vp, C.c_int, C.POINTER(C.c_longlong), PD, PD]
     lib.cm_system_kernel_stats.restype = C.c_int
     for f in ("cm_system_destroy", "cm_system_set_rhs", "cm_system_set_x", "cm_system_get_x",
@@ -559,15 +561,17 @@ class System:
 
     KERNEL_CATS = ("A", "F1", "F2", "LU", "VEC", "SETUP", "FIN")
 
-    def profile(self, on=True, cats=None):
-        """Bracket launches with HIP events: all categories, or only `cats`."""
+    def profile(self, on=True, cats=None, sample=0):
+        """Bracket launches with HIP events: all categories, or only `cats`;
+        sample > 0 brackets only the first `sample` passes of each solve (the
+        rest replay from graphs)."""
         if not on:
             mask = 0
         elif cats is None:
             mask = (1 << len(self.KERNEL_CATS)) - 1
         else:
             mask = sum(1 << self.KERNEL_CATS.index(c) for c in cats)
-        _check(self._lib.cm_system_profile(self._h, mask))
+        _check(self._lib.cm_system_profile_sampled(self._h, mask, int(sample)))
 
     def kernel_stats(self):
         """{category: (launches, total_ms, total_bytes)} for the profiled solves."""

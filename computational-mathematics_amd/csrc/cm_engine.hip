@@ -686,6 +686,9 @@ enum KernelCat { KC_A = 0, KC_F1 = 1, KC_F2 = 2, KC_LU = 3, KC_VEC = 4, KC_SETUP
 struct Prof {
     bool on = false;
     unsigned mask = 0;  // categories whose launches are bracketed
+    int sample = 0;     // > 0: only the first `sample` passes of a solve are bracketed (and launched
+                        // one by one); the others replay from the captured graphs
+    int sampled = 0;    // passes of the current solve launched with brackets so far
     int pass = -1;      // pass being enqueued (-1: setup)
     bool p_update = false;
     struct Rec {
@@ -1060,9 +1063,15 @@ struct Krylov {
     int capture(hipGraphExec_t &out, int passes)
     {
         hipGraph_t graph = nullptr;
+        const bool prof_on = prof.on;  // captured passes carry no timing events
+        const int pass = prof.pass;
+        prof.on = false;
         CM_HIP(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
         for (int i = 0; i < passes; i++) enqueue_iteration();
-        CM_HIP(hipStreamEndCapture(stream, &graph));
+        const hipError_t cerr = hipStreamEndCapture(stream, &graph);
+        prof.on = prof_on;
+        prof.pass = pass;
+        CM_HIP(cerr);
         const hipError_t err = hipGraphInstantiate(&out, graph, nullptr, nullptr, 0);
         (void)hipGraphDestroy(graph);
         CM_HIP(err);
@@ -1071,9 +1080,16 @@ struct Krylov {
 
     int enqueue_passes(int size)
     {
-        if (prof.on || !use_graph) {
+        if (!use_graph || (prof.on && prof.sample <= 0)) {
             for (int i = 0; i < size; i++) enqueue_iteration();
             return CM_OK;
+        }
+        if (prof.on && prof.sampled < prof.sample) {  // the solve's first passes, bracketed
+            const int e = std::min(size, prof.sample - prof.sampled);
+            for (int i = 0; i < e; i++) enqueue_iteration();
+            prof.sampled += e;
+            size -= e;
+            if (size == 0) return CM_OK;
         }
         if (!g1) {
             CM_TRY(capture(g1, 1));
@@ -1106,6 +1122,7 @@ struct Krylov {
         hst[0] = init;
         CM_HIP(hipMemcpyAsync(st, &hst[0], sizeof(SolveState), hipMemcpyHostToDevice, stream));
         prof.pass = -1;
+        prof.sampled = 0;
         return CM_OK;
     }
 
@@ -2812,13 +2829,20 @@ long long cm_system_nnz(cm_system h, char which)
 
 int cm_system_profile(cm_system h, int mask)
 {
-    if (!h) return fail(CM_ERR_ARG, "null handle");
+    return cm_system_profile_sampled(h, mask, 0);
+}
+
+int cm_system_profile_sampled(cm_system h, int mask, int passes)
+{
+    if (!h || passes < 0) return fail(CM_ERR_ARG, "bad argument");
     System &S = h->sys;
     Prof *ps[] = {&S.k64->prof, S.k32 ? &S.k32->prof : nullptr, S.k16 ? &S.k16->prof : nullptr};
     for (Prof *p : ps)
         if (p) {
             p->on = (mask != 0);
             p->mask = (unsigned)mask;
+            p->sample = passes;
+            p->sampled = 0;
             p->reset_stats();
         }
     return CM_OK;

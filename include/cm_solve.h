@@ -200,6 +200,11 @@ int cm_system_apply(cm_system sys, const double *v, double *out);
  * one-workgroup reduction finalizers.  total_bytes
  * is the algorithmic (compulsory) traffic of the counted launches. */
 int cm_system_profile(cm_system sys, int mask);
+/* As cm_system_profile, but only the first `passes` passes of each solve
+ * are bracketed (and launched kernel by kernel); the other passes replay
+ * from the captured hipGraphs, so timing costs the solve little.  passes = 0
+ * brackets every pass (cm_system_profile). */
+int cm_system_profile_sampled(cm_system sys, int mask, int passes);
 int cm_system_kernel_stats(cm_system sys, int category, long long *launches, double *total_ms,
                            double *total_bytes);
 

@@ -370,6 +370,30 @@ def test_slot_major_format_refuses_long_rows():
     sys_.close()
 
 
+def test_sampled_profiling_counts_and_results():
+    """Sampled profiling brackets only the first passes of each solve and
+    replays the rest from graphs: the counted launches match, and the
+    solution is bit-identical to an unprofiled solve."""
+    A = _system("cd2d")
+    b = cmamd.csr_mv_host(A, cmamd.gen_uniform(2, A.n))
+    F = cmamd.afiim(A, 0.1)
+    sys_ = cmamd.System("afiim", A, F)
+    sys_.set_rhs(b)
+    sys_.set_x(None)
+    rv0, it0, info0, _ = sys_.solve(tol=1e-8, nmax=500)
+    x0 = sys_.get_x()
+    for _ in range(2):
+        sys_.profile(True, cats=("F1",), sample=5)
+        sys_.set_x(None)
+        rv, it, info, _ = sys_.solve(tol=1e-8, nmax=500)
+        st = sys_.kernel_stats()
+        assert (rv, it, info) == (rv0, it0, info0)
+        assert np.array_equal(sys_.get_x(), x0)
+        assert st["F1"][0] == 2 * 5 and st["A"][0] == 0  # two factor applies per pass, 5 passes
+    sys_.profile(False)
+    sys_.close()
+
+
 def test_graph_replays_match_eager_single():
     """Single-RHS passes replayed from captured hipGraphs against kernel-by-
     kernel launches: bit-identical traces and solutions over several replays,

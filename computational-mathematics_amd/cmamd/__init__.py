@@ -281,7 +281,7 @@ SOLVE_SYMBOLS = [
     "cm_system_block_set", "cm_system_block_get", "cm_system_block_solve", "cm_system_block_solve_mixed",
     "cm_system_graphs", "cm_system_block_graphs", "cm_system_block_release",
     "cm_system_block_time_spmm", "cm_batch_solve", "cm_system_autotune", "cm_system_block_autotune",
-    "cm_system_spmv_format", "cm_system_spmv_formats", "cm_system_profile_sampled", "cm_last_error", "cm_device_count",
+    "cm_system_spmv_format", "cm_system_spmv_formats", "cm_system_profile_sampled", "cm_system_block_dia", "cm_last_error", "cm_device_count",
     "cm_server_create", "cm_server_configure", "cm_server_destroy", "cm_server_add_system", "cm_server_submit", "cm_server_wait",
     "cm_server_stats",
 ]
@@ -357,6 +357,8 @@ def _type_solve(lib):
     lib.cm_system_profile.restype = C.c_int
     lib.cm_system_profile_sampled.argtypes = [vp, C.c_int, C.c_int]
     lib.cm_system_profile_sampled.restype = C.c_int
+    lib.cm_system_block_dia.argtypes = [vp, C.c_int]
+    lib.cm_system_block_dia.restype = C.c_int
     lib.cm_system_kernel_stats.argtypes = [vp, C.c_int, C.POINTER(C.c_longlong), PD, PD]
     lib.cm_system_kernel_stats.restype = C.c_int
     for f in ("cm_system_destroy", "cm_system_set_rhs", "cm_system_set_x", "cm_system_get_x",
@@ -475,6 +477,10 @@ class System:
     def graphs(self, on=True):
         """Single-RHS passes from captured hipGraphs (on) or kernel by kernel."""
         _check(self._lib.cm_system_graphs(self._h, 1 if on else 0))
+
+    def block_dia(self, on=True):
+        """Block products in the diagonal format where a matrix suits it (default on)."""
+        _check(self._lib.cm_system_block_dia(self._h, 1 if on else 0))
 
     def block_graphs(self, on=True):
         _check(self._lib.cm_system_block_graphs(self._h, 1 if on else 0))

@@ -262,6 +262,76 @@ __global__ __launch_bounds__(BS) void spmm_short(int cnt, const int *__restrict_
     block_colgroup_partials<Epi::K, KB, CPL>(acc, red, partials, slot0 + blockIdx.x);
 }
 
+// Diagonals plus a CSR remainder (the single-RHS spmv_dia layout: dval[d *
+// ldn + i] = A(i, i + off[d]), remainder rptr / rcol / rval, rptr null when
+// empty) over KB columns: LPRW lanes per row, each with CPL columns, rows
+// walked in contiguous chunks per workgroup like spmm_short.  A diagonal's
+// value is one load per row (shared by its lanes), its x segment the row's
+// KB columns at row i + off[d], so consecutive rows read consecutive x
+// segments and no column index is read for the diagonal part.  Row sums in
+// diagonal order, then the remainder.
+template <class V, class T, class Acc, int KB, class Epi>
+__global__ __launch_bounds__(BS) void spmm_dia(int n, int ldn, int nd, const int *__restrict__ off,
+                                               const V *__restrict__ dval, const int *__restrict__ rptr,
+                                               const int *__restrict__ rcol, const V *__restrict__ rval,
+                                               const float *__restrict__ rowscale, const T *__restrict__ x, Epi epi,
+                                               const BlockFlags *fl, double *partials, int slot0)
+{
+    if (fl && fl->all_done) return;
+    constexpr int CPL = ColSplit<T, KB>::CPL, LPRW = ColSplit<T, KB>::LPRW, KK = Epi::K > 0 ? Epi::K : 1;
+    constexpr int CH = 4;  // diagonals in flight per step
+    __shared__ double red[(BS / WAVE) * KK * KB];
+    double acc[CPL][KK];
+#pragma unroll
+    for (int i = 0; i < CPL; i++)
+#pragma unroll
+        for (int k = 0; k < KK; k++) acc[i][k] = 0.0;
+    const int c0 = (threadIdx.x % LPRW) * CPL;
+    constexpr int SR = BS / LPRW;
+    const int chunk = ((n + (int)gridDim.x - 1) / (int)gridDim.x + SR - 1) / SR * SR;
+    const int r_begin = xcd_block(blockIdx.x, gridDim.x) * chunk, r_end = min(n, r_begin + chunk);
+    for (int r = r_begin + (int)threadIdx.x / LPRW; r < r_end; r += SR) {
+#pragma clang fp contract(off)
+        Acc s[CPL];
+#pragma unroll
+        for (int i = 0; i < CPL; i++) s[i] = 0;
+        for (int d0 = 0; d0 < nd; d0 += CH) {
+            Acc v[CH];
+            T xv[CH][CPL];
+#pragma unroll
+            for (int q = 0; q < CH; q++) {
+                const int d = d0 + q;
+                v[q] = 0;
+#pragma unroll
+                for (int i = 0; i < CPL; i++) xv[q][i] = 0;
+                if (d < nd) {
+                    const int c = r + off[d];
+                    v[q] = load_val<V, Acc>(dval + (size_t)d * ldn, r);
+                    if ((unsigned)c < (unsigned)n) ld_cols<T, CPL>(x + (size_t)c * KB + c0, xv[q]);
+                }
+            }
+#pragma unroll
+            for (int q = 0; q < CH; q++)
+                if (d0 + q < nd) {
+#pragma unroll
+                    for (int i = 0; i < CPL; i++) s[i] += v[q] * (Acc)xv[q][i];
+                }
+        }
+        if (rptr)
+            for (int k = rptr[r]; k < rptr[r + 1]; k++) {
+                T xr[CPL];
+                ld_cols<T, CPL>(x + (size_t)rcol[k] * KB + c0, xr);
+                const Acc vk = load_val<V, Acc>(rval, k);
+#pragma unroll
+                for (int i = 0; i < CPL; i++) s[i] += vk * (Acc)xr[i];
+            }
+        const Acc sc = rowscale ? (Acc)rowscale[r] : (Acc)1;
+#pragma unroll
+        for (int i = 0; i < CPL; i++) epi.row((size_t)r * KB + c0 + i, c0 + i, rowscale ? s[i] * sc : s[i], acc[i]);
+    }
+    block_colgroup_partials<Epi::K, KB, CPL>(acc, red, partials, slot0 + blockIdx.x);
+}
+
 // Long rows: one wave per row; lane = column group (LPRW of them) + LPRW x
 // nonzero group (64 / LPRW groups striding over the row), groups summed with
 // xor shuffles.

@@ -34,7 +34,6 @@
 #include <memory>
 #include <string>
 #include <type_traits>
-#include <unordered_map>
 #include <vector>
 
 #include "cm_kernels.hpp"
@@ -276,6 +275,7 @@ struct DevCsr {
     std::shared_ptr<DiaIndex> dia;     // diagonal index (shared by the copies), if built
     V *dval = nullptr, *rval = nullptr;  // this copy's diagonal and remainder values, if built
     bool use_dia = false;              // products through spmv_dia
+    bool block_dia = false;            // block products (SpMM) through spmm_dia
     int nlev = 0;
     int *lrows = nullptr;
     std::vector<int> loff;
@@ -316,7 +316,7 @@ struct DevCsr {
         rowscale = nullptr;
         slots.reset();
         dia.reset();
-        use_slot = slot16 = use_dia = false;
+        use_slot = slot16 = use_dia = block_dia = false;
     }
     ~DevCsr() { release(); }
 };
@@ -519,12 +519,13 @@ static int build_dia(DevCsr<double> &m, hipStream_t s)
     CM_HIP(hipMemcpyAsync(ptr.data(), m.ptr, ptr.size() * sizeof(int), hipMemcpyDeviceToHost, s));
     if (m.nnz) CM_HIP(hipMemcpyAsync(col.data(), m.col, col.size() * sizeof(int), hipMemcpyDeviceToHost, s));
     CM_HIP(hipStreamSynchronize(s));
-    std::unordered_map<int, long long> cnt;
+    // entries per diagonal, indexed by offset + n - 1 (offsets span 1 - n .. n - 1)
+    std::vector<int> cnt(2 * (size_t)n - 1, 0);
     for (int i = 0; i < n; i++)
-        for (int k = ptr[i]; k < ptr[i + 1]; k++) cnt[col[k] - i]++;
+        for (int k = ptr[i]; k < ptr[i + 1]; k++) cnt[(size_t)(col[k] - i + n - 1)]++;
     std::vector<std::pair<long long, int>> cand;
-    for (const auto &kv : cnt)
-        if ((double)kv.second >= DIA_FILL * n) cand.push_back({kv.second, kv.first});
+    for (size_t o = 0; o < cnt.size(); o++)
+        if ((double)cnt[o] >= DIA_FILL * n) cand.push_back({cnt[o], (int)((long long)o - (n - 1))});
     if (cand.empty()) return CM_OK;
     std::sort(cand.begin(), cand.end(), [](const auto &a, const auto &b) {
         return a.first != b.first ? a.first > b.first : a.second < b.second;
@@ -535,14 +536,15 @@ static int build_dia(DevCsr<double> &m, hipStream_t s)
     std::sort(off.begin(), off.end());
     const int nd = (int)off.size();
     const int ldn = (n + DIA_ALIGN - 1) / DIA_ALIGN * DIA_ALIGN;
-    std::unordered_map<int, int> slot;
-    for (int d = 0; d < nd; d++) slot[off[d]] = d;
+    // diagonal index by offset (reusing the count array): -1 for the others
+    std::fill(cnt.begin(), cnt.end(), -1);
+    for (int d = 0; d < nd; d++) cnt[(size_t)(off[d] + n - 1)] = d;
     std::vector<int> dsrc((size_t)nd * ldn, -1), rptr(n + 1, 0), rcol, rsrc;
     for (int i = 0; i < n; i++) {
         for (int k = ptr[i]; k < ptr[i + 1]; k++) {
-            const auto it = slot.find(col[k] - i);
-            if (it != slot.end()) {
-                dsrc[(size_t)it->second * ldn + i] = k;
+            const int d = cnt[(size_t)(col[k] - i + n - 1)];
+            if (d >= 0) {
+                dsrc[(size_t)d * ldn + i] = k;
             } else {
                 rcol.push_back(col[k]);
                 rsrc.push_back(k);
@@ -1330,7 +1332,7 @@ static void follow_blocks(DevCsr<V> &dst, const DevCsr<double> &src)
         (void)hipFree(dst.rval);
         dst.dval = dst.rval = nullptr;
         dst.dia = src.dia;
-        dst.use_dia = false;
+        dst.use_dia = dst.block_dia = false;
     }
 }
 
@@ -1536,6 +1538,7 @@ struct BlockKrylov {
     static constexpr int FIN_ONE = 256, FIN_PER = 64;
     static int short_grid(const RowBins &bn) { return std::min((bn.ns + SROWS - 1) / SROWS, SPMM_CAP); }
     static int long_grid(const RowBins &bn) { return std::min((bn.nl + (BS / WAVE) - 1) / (BS / WAVE), SPMM_CAP); }
+    static int dia_grid(int nrows) { return std::min((nrows + SROWS - 1) / SROWS, SPMM_CAP); }
 
     // device bytes of a block solver of this type for an n-row system whose
     // three matrices have (short + long) row-bin workgroups `slots`
@@ -1568,7 +1571,7 @@ struct BlockKrylov {
         CM_HIP(hipHostMalloc((void **)&hst, 3 * KB * sizeof(SolveState), hipHostMallocDefault));
         CM_HIP(hipMalloc((void **)&fl, sizeof(BlockFlags)));
         CM_HIP(hipHostMalloc((void **)&hfl, 3 * sizeof(BlockFlags), hipHostMallocDefault));
-        size_t slots = (size_t)vgrid();
+        size_t slots = std::max((size_t)vgrid(), (size_t)dia_grid(n));
         for (const RowBins *bn : {&bA, &bG, &bH})
             slots = std::max(slots, (size_t)(short_grid(*bn) + long_grid(*bn)));
         partial_slots = std::max(slots, (size_t)MAX_RED_BLOCKS);
@@ -1586,6 +1589,14 @@ struct BlockKrylov {
     {
         using Acc = typename std::conditional<std::is_same<T, double>::value, double, float>::type;
         const BlockFlags *gf = gate ? fl : nullptr;
+        if (m.block_dia && m.dval) {
+            const DiaIndex &D = *m.dia;
+            const int gd = dia_grid(n);
+            spmm_dia<V, T, Acc, KB, Epi><<<gd, BS, 0, stream>>>(n, D.ldn, D.nd, D.off, m.dval, D.rptr, D.rcol, m.rval,
+                                                               m.rowscale, xin, epi, gf, partials, 0);
+            if constexpr (Epi::K > 0) finalize<Epi::K>(gd, epi, fst, gate, gate ? 1 : 0);
+            return;
+        }
         const int gs = short_grid(bn), gl = long_grid(bn);
         if (gs > 0)
             spmm_short<V, T, Acc, KB, Epi><<<gs, BS, 0, stream>>>(bn.ns, bn.shortr, m.ptr, m.col, m.val, m.rowscale,
@@ -1617,8 +1628,14 @@ struct BlockKrylov {
         (void)hipEventDestroy(e0);
         (void)hipEventDestroy(e1);
         *avg_us = 1e3 * ms / reps;
-        *bytes = (double)m.nnz * (sizeof(V) + sizeof(int)) + 4.0 * (m.n + 1) + 4.0 * (bn.ns + bn.nl) +
-                 2.0 * (double)ne() * sizeof(T) + (m.rowscale ? 4.0 * m.n : 0.0);
+        if (m.block_dia && m.dval) {  // diagonals (zeros included) and remainder once, X and Y once
+            const DiaIndex &D = *m.dia;
+            *bytes = (double)D.nd * m.n * sizeof(V) + 4.0 * D.nd + (double)D.nrem * (sizeof(V) + sizeof(int)) +
+                     (D.rptr ? 4.0 * (m.n + 1) : 0.0) + 2.0 * (double)ne() * sizeof(T) + (m.rowscale ? 4.0 * m.n : 0.0);
+        } else {
+            *bytes = (double)m.nnz * (sizeof(V) + sizeof(int)) + 4.0 * (m.n + 1) + 4.0 * (bn.ns + bn.nl) +
+                     2.0 * (double)ne() * sizeof(T) + (m.rowscale ? 4.0 * m.n : 0.0);
+        }
         CM_HIP(hipGetLastError());
         return CM_OK;
     }
@@ -1651,7 +1668,7 @@ struct BlockKrylov {
 
     int fit_partials()
     {
-        size_t slots = (size_t)vgrid();
+        size_t slots = std::max((size_t)vgrid(), (size_t)dia_grid(n));
         for (const RowBins *bn : {&bA, &bG, &bH})
             slots = std::max(slots, (size_t)(short_grid(*bn) + long_grid(*bn)));
         if (slots <= partial_slots) return CM_OK;
@@ -1665,11 +1682,16 @@ struct BlockKrylov {
         return CM_OK;
     }
 
-    int tune(int reps, int *chosen /* A, G, H */)
+    void drop_graphs()
     {
         if (g1) (void)hipGraphExecDestroy(g1);
         if (gN) (void)hipGraphExecDestroy(gN);
         g1 = gN = nullptr;
+    }
+
+    int tune(int reps, int *chosen /* A, G, H */)
+    {
+        drop_graphs();
         CM_TRY(tune_one(*A, bA, reps, &chosen[0]));
         CM_TRY(tune_one(*G, bG, reps, &chosen[1]));
         CM_TRY(tune_one(*Hm, bH, reps, &chosen[2]));
@@ -1821,6 +1843,7 @@ struct BlockBase {
                             int *outer, int *inner_total, int *info) = 0;
     virtual void set_stream(hipStream_t s) = 0;
     virtual void set_graphs(bool on) = 0;
+    virtual void drop_graphs() = 0;
     virtual size_t bytes() const = 0;
     virtual int time_spmm(char which, int bits, int reps, double *avg_us, double *bytes) = 0;
     virtual int tune(int bits, int reps, int *chosen) = 0;
@@ -1881,6 +1904,14 @@ struct BlockSolver : BlockBase {
         b64.use_graph = on;
         if (b32) b32->use_graph = on;
         if (b16) b16->use_graph = on;
+    }
+
+    void drop_graphs() override
+    {
+        (void)hipStreamSynchronize(stream);
+        b64.drop_graphs();
+        if (b32) b32->drop_graphs();
+        if (b16) b16->drop_graphs();
     }
 
     template <class KI, class BI>
@@ -2114,6 +2145,11 @@ static int make_block(std::unique_ptr<BlockBase> (&cache)[5], BlockBase *&cur, i
     return CM_OK;
 }
 
+// Block products in the diagonal format: a matrix qualifies when its
+// diagonal index exists and leaves at most BLOCK_DIA_REM of the nonzeros to
+// the CSR remainder (long irregular rows stay with the row-binned kernels).
+constexpr double BLOCK_DIA_REM = 0.15;
+
 // Results of the last mixed-precision solve.
 struct MixedStats {
     int outer = 0, inner_total = 0, info = 0;
@@ -2134,6 +2170,7 @@ struct System {
     std::unique_ptr<BlockBase> blks[5];
     BlockBase *blk = nullptr;
     bool block_graphs = true;
+    bool block_dia = true;  // block products in the diagonal format where a matrix suits it
     bool graphs = true;  // single-RHS passes from captured graphs
     size_t bytes() const
     {
@@ -2155,6 +2192,46 @@ struct System {
         if (stream) (void)hipStreamDestroy(stream);
     }
 };
+
+// Decides, per SpMV matrix (A, G, H, G^T) and precision copy, whether block
+// products use the diagonal format (S.block_dia and at most BLOCK_DIA_REM of
+// the nonzeros in the remainder), building the diagonal index and values on
+// first use.  Idempotent; ends with the system stream synchronized, so block
+// solves on other streams see the built arrays.
+static int prepare_block_dia(System &S)
+{
+    K64 &k = *S.k64;
+    DevCsr<double> *ms[] = {&k.A, &k.G, &k.H, &k.GT};
+    for (int q = 0; q < 4; q++) {
+        DevCsr<double> &m = *ms[q];
+        if (!m.present()) continue;
+        bool ok = false;
+        if (S.block_dia) {
+            CM_TRY(build_dia(m, k.stream));
+            ok = m.dia && (double)m.dia->nrem <= BLOCK_DIA_REM * (double)m.nnz;
+            if (ok) CM_TRY(build_dia_vals(m, k.stream));
+        }
+        m.block_dia = ok;
+        auto low = [&](auto &c) -> int {
+            follow_blocks(c, m);
+            if (ok) CM_TRY(build_dia_vals(c, k.stream));
+            c.block_dia = ok;
+            return CM_OK;
+        };
+        if (S.k32) {
+            DevCsr<float> *c32[] = {&S.k32->A, &S.k32->G, &S.k32->H, &S.k32->GT};
+            CM_TRY(low(*c32[q]));
+        }
+        if (S.k16) {
+            if (q == 0) CM_TRY(low(S.k16->A));
+            if (q == 1) CM_TRY(low(S.k16->G));
+            if (q == 2) CM_TRY(low(S.k16->H));
+            if (q == 3) CM_TRY(low(S.k16->GT));
+        }
+    }
+    CM_HIP(hipStreamSynchronize(k.stream));
+    return CM_OK;
+}
 
 static int validate(const HostSys &h)
 {
@@ -2557,11 +2634,22 @@ int cm_system_block_set(cm_system h, int k, const double *B, const double *X0)
 {
     if (!h || !B) return fail(CM_ERR_ARG, "null argument");
     System &S = h->sys;
+    CM_TRY(prepare_block_dia(S));
     CM_TRY(make_block(S.blks, S.blk, k, *S.k64));
     S.blk->k32 = S.k32.get();  // mixed precision may have been enabled since
     S.blk->k16 = S.k16.get();
     S.blk->set_graphs(S.block_graphs);
     return S.blk->set(k, B, X0);
+}
+
+int cm_system_block_dia(cm_system h, int on)
+{
+    if (!h) return fail(CM_ERR_ARG, "null handle");
+    System &S = h->sys;
+    S.block_dia = on != 0;
+    for (auto &q : S.blks)  // captured block passes carry the old kernels
+        if (q) q->drop_graphs();
+    return prepare_block_dia(S);
 }
 
 int cm_system_graphs(cm_system h, int on)

@@ -272,6 +272,7 @@ struct Server {
         }
         const auto t0 = Clock::now();
         for (Request *r : batch) r->t_start = t0;
+        CM_TRY(prepare_block_dia(S));
         CM_TRY(make_block(S.blks, S.blk, k, *S.k64));
         BlockBase &blk = *S.blk;
         blk.k32 = S.k32.get();
@@ -501,6 +502,7 @@ int cm_server_add_system(cm_server h, cm_system sys, int *sys_id)
     System &Y = sys->sys;
     if (Y.variant == 2 && Y.k64->mode != 'E') return fail(CM_ERR_UNSUPPORTED, "served systems use apply mode E");
     if (S.precision != 64) CM_TRY(cm_system_enable_mixed(sys, S.precision));
+    CM_TRY(prepare_block_dia(Y));  // diagonal block products: index and values built before serving
     std::lock_guard<std::mutex> lk(S.mu);
     for (const auto &q : S.systems)
         if (q.sys == &Y) return fail(CM_ERR_ARG, "system already registered");

@@ -181,6 +181,13 @@ int cm_system_spmv_formats(cm_system sys, int *formats);
  * for A, G and H the short-row limit among 16, 32, 64, 128 nonzeros that
  * times fastest; short_max[3] receives the choices. */
 int cm_system_block_autotune(cm_system sys, int bits, int reps, int *short_max);
+/* Block products (SpMM) in the diagonal format (on, the default) for every
+ * SpMV matrix whose diagonals held by at least 60% of the rows leave at most
+ * 15% of its nonzeros to a CSR remainder; the other matrices, and all of
+ * them when off, use the row-binned kernels.  The diagonal index and values
+ * are built on the first block solve (or here) and shared with the
+ * single-RHS diagonal format. */
+int cm_system_block_dia(cm_system sys, int on);
 /* Frees the block workspaces (one per block width used so far). */
 int cm_system_block_release(cm_system sys);
 

@@ -60,6 +60,41 @@ def test_block_afiim_columns(k):
     _check_columns("afiim", A, F, B, X, rv, it, info, 500)
 
 
+@pytest.mark.parametrize("k", [1, 5, 16])
+def test_block_dia_matches_row_binned(k):
+    """Block products in the diagonal format (the default for stencil and
+    factor matrices) against the row-binned kernels (block_dia off): both meet
+    the oracle checks in FP64, both FP32- and FP16-inner refinements reach
+    the tolerance, and the two agree to 1e-6."""
+    A = cmamd.gen_convdiff(3, 20, 2.0, seed=17)
+    F = cmamd.afiim(A, 0.1)
+    B = _rhs(A, k, 70)
+    out = {}
+    for on in (True, False):
+        sys_ = cmamd.System("afiim", A, F)
+        sys_.enable_mixed(32)
+        sys_.enable_mixed(16)
+        sys_.block_dia(on)
+        sys_.block_set(B)
+        rv, it, info = sys_.block_solve(tol=1e-8, nmax=500)
+        X = sys_.block_get()
+        _check_columns("afiim", A, F, B, X, rv, it, info, 500)
+        res = {64: X}
+        for bits in (32, 16):
+            sys_.block_set(B)
+            rv, outer, inner, info = sys_.block_solve_mixed(bits, tol=1e-8, inner_tol=1e-4, nmax=500)
+            Xm = sys_.block_get()
+            assert np.all(info == 0)
+            for j in range(k):
+                assert np.linalg.norm(B[:, j] - cmamd.csr_mv_host(A, Xm[:, j])) / np.linalg.norm(B[:, j]) <= 1e-8
+            res[bits] = Xm
+        out[on] = res
+        sys_.close()
+    for bits in (64, 32, 16):
+        for j in range(k):
+            assert _rel(out[True][bits][:, j], out[False][bits][:, j]) <= 1e-6, (bits, j)
+
+
 def test_block_long_rows():
     # 27-point stencil with long-range couplings: 154 rows longer than the
     # short-row bin (up to 623 nonzeros)

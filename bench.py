@@ -267,6 +267,7 @@ def bench_block(args, w):
         tuned = sys_.autotune(reps=10)
     if args.no_graphs:
         sys_.block_graphs(False)
+        sys_.graphs(False)
     steps = max(1, args.steps)
 
     def timed(fn, reps):
@@ -407,6 +408,7 @@ def bench_serve(args, w):
     if args.no_graphs:
         for h in handles:
             h.block_graphs(False)
+            h.graphs(False)
 
     def check(i, X):
         """(worst true rel-res, worst error vs x*) of request i's columns."""

@@ -2148,7 +2148,7 @@ static int make_block(std::unique_ptr<BlockBase> (&cache)[5], BlockBase *&cur, i
 // Block products in the diagonal format: a matrix qualifies when its
 // diagonal index exists and leaves at most BLOCK_DIA_REM of the nonzeros to
 // the CSR remainder (long irregular rows stay with the row-binned kernels).
-constexpr double BLOCK_DIA_REM = 0.15;
+constexpr double BLOCK_DIA_REM = 0.3;
 
 // Results of the last mixed-precision solve.
 struct MixedStats {

@@ -183,7 +183,7 @@ int cm_system_spmv_formats(cm_system sys, int *formats);
 int cm_system_block_autotune(cm_system sys, int bits, int reps, int *short_max);
 /* Block products (SpMM) in the diagonal format (on, the default) for every
  * SpMV matrix whose diagonals held by at least 60% of the rows leave at most
- * 15% of its nonzeros to a CSR remainder; the other matrices, and all of
+ * 30% of its nonzeros to a CSR remainder; the other matrices, and all of
  * them when off, use the row-binned kernels.  The diagonal index and values
  * are built on the first block solve (or here) and shared with the
  * single-RHS diagonal format. */

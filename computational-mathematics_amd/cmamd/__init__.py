@@ -464,7 +464,14 @@ class System:
 
     def block_set(self, B, X0=None):
         """Resident block of k right-hand sides (B: n x k), initial guesses
-        X0 (n x k) or zeros."""
+        X0 (n x k) or zeros.  B = None keeps the resident right-hand sides
+        and only resets the initial guesses."""
+        if B is None:
+            if getattr(self, "_bk", 0) < 1:
+                raise ValueError("no resident block: pass B")
+            px = None if X0 is None else _dp(np.asfortranarray(np.asarray(X0, np.float64).reshape(self.n, self._bk)))
+            _check(self._lib.cm_system_block_set(self._h, self._bk, None, px))
+            return
         B = np.asfortranarray(np.asarray(B, np.float64).reshape(self.n, -1))
         self._bk = B.shape[1]
         if X0 is None:

@@ -1931,8 +1931,10 @@ struct BlockSolver : BlockBase {
         ncols = nc;
         b64.ncols = nc;
         const int g = (int)std::min<size_t>(((size_t)n * KB + BS - 1) / BS, 8192);
-        CM_HIP(hipMemcpyAsync(cm, B, (size_t)n * nc * sizeof(double), hipMemcpyHostToDevice, stream));
-        k_interleave<double, KB><<<g, BS, 0, stream>>>(n, nc, cm, b64.b);
+        if (B) {  // B null: the resident right-hand sides are kept, only x is reset
+            CM_HIP(hipMemcpyAsync(cm, B, (size_t)n * nc * sizeof(double), hipMemcpyHostToDevice, stream));
+            k_interleave<double, KB><<<g, BS, 0, stream>>>(n, nc, cm, b64.b);
+        }
         if (X0) {
             CM_HIP(hipMemcpyAsync(cm, X0, (size_t)n * nc * sizeof(double), hipMemcpyHostToDevice, stream));
             k_interleave<double, KB><<<g, BS, 0, stream>>>(n, nc, cm, b64.x);
@@ -2632,8 +2634,13 @@ int cm_system_solve_mixed(cm_system h, int factor_bits, double tol, double inner
 
 int cm_system_block_set(cm_system h, int k, const double *B, const double *X0)
 {
-    if (!h || !B) return fail(CM_ERR_ARG, "null argument");
+    if (!h) return fail(CM_ERR_ARG, "null argument");
     System &S = h->sys;
+    if (!B) {  // keep the resident block of right-hand sides, reset x (to X0, or zero)
+        if (!S.blk || S.blk->ncols != k)
+            return fail(CM_ERR_ARG, "no resident block of this width: pass the right-hand sides");
+        return S.blk->set(k, nullptr, X0);
+    }
     CM_TRY(prepare_block_dia(S));
     CM_TRY(make_block(S.blks, S.blk, k, *S.k64));
     S.blk->k32 = S.k32.get();  // mixed precision may have been enabled since

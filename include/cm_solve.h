@@ -115,7 +115,10 @@ int cm_system_solve_mixed(cm_system sys, int factor_bits, double tol, double inn
  *                first); per-column resval[k] and info[k] as in
  *                cm_system_solve_mixed; outer = refinement steps, inner_total
  *                = inner passes summed over steps (the longest column);
- *   block_get:   the k solutions, column-major n x k. */
+ *   block_get:   the k solutions, column-major n x k.
+ * block_set with B = NULL keeps the resident block of k right-hand sides
+ * (set before with the same k) and only resets x (to X0, or zero), so a
+ * block can be re-solved without moving B over PCIe again. */
 int cm_system_block_set(cm_system sys, int k, const double *B, const double *X0);
 int cm_system_block_solve(cm_system sys, double tol, int NMAX, double *resval, int *iter, int *info);
 int cm_system_block_solve_mixed(cm_system sys, int factor_bits, double tol, double inner_tol, int NMAX,

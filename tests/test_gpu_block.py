@@ -95,6 +95,28 @@ def test_block_dia_matches_row_binned(k):
             assert _rel(out[True][bits][:, j], out[False][bits][:, j]) <= 1e-6, (bits, j)
 
 
+def test_block_resolve_with_resident_rhs():
+    """block_set(None) keeps the resident right-hand sides and resets x: the
+    second solve repeats the first bit for bit; a different width without B
+    is refused."""
+    A = cmamd.gen_convdiff(2, 60, 1.0, seed=23)
+    F = cmamd.afiim(A, 0.1)
+    B = _rhs(A, 4, 90)
+    sys_ = cmamd.System("afiim", A, F)
+    sys_.enable_mixed(32)
+    sys_.block_set(B)
+    r1 = sys_.block_solve_mixed(32, tol=1e-8, inner_tol=1e-4, nmax=500)
+    X1 = sys_.block_get()
+    sys_.block_set(None)
+    r2 = sys_.block_solve_mixed(32, tol=1e-8, inner_tol=1e-4, nmax=500)
+    X2 = sys_.block_get()
+    assert np.array_equal(X1, X2) and np.array_equal(r1[0], r2[0]) and r1[1:3] == r2[1:3]
+    sys_._bk = 5
+    with pytest.raises(cmamd.EngineError):
+        sys_.block_set(None)
+    sys_.close()
+
+
 def test_block_long_rows():
     # 27-point stencil with long-range couplings: 154 rows longer than the
     # short-row bin (up to 623 nonzeros)

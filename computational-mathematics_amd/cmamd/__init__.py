@@ -506,7 +506,7 @@ class System:
         _check(self._lib.cm_system_autotune(self._h, reps, _ip(t)))
         return dict(zip(("A", "G", "H", "GT", "L", "U"), (int(v) for v in t)))
 
-    SPMV_FORMATS = ("staged", "staged16", "slot", "slot16", "dia")
+    SPMV_FORMATS = ("staged", "staged16", "slot", "slot16", "dia", "dia2")
     MATRICES = ("A", "G", "H", "GT", "L", "U")
 
     def spmv_format(self, bits, which, fmt):

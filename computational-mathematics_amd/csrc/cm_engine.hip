@@ -241,11 +241,14 @@ struct DiaIndex {
 // SpMV formats of a resident matrix copy: the staged row blocks with 32-bit
 // or 16-bit column indices, the slot-major blocks with either, or the
 // diagonals with a CSR remainder.
-enum SpmvFormat { FMT_STAGED = 0, FMT_STAGED16 = 1, FMT_SLOT = 2, FMT_SLOT16 = 3, FMT_DIA = 4 };
+enum SpmvFormat { FMT_STAGED = 0, FMT_STAGED16 = 1, FMT_SLOT = 2, FMT_SLOT16 = 3, FMT_DIA = 4, FMT_DIA2 = 5 };
 constexpr double DIA_FILL = 0.6;  // a diagonal is stored densely when at least this share of rows has it
 
 // Rows per thread of spmv_dia: FP16 values are read two rows at a time (one
-// 4-byte load per diagonal); FP32 and FP64 one row per thread.
+// 4-byte load per diagonal); FP32 and FP64 one row per thread.  The second
+// diagonal variant (FMT_DIA2) doubles the rows per thread and keeps fewer
+// diagonals in flight (6 instead of 8): faster on matrices with many
+// diagonals (the 17 of the 128^3 PDFIIM H factor: 3.7%), slower on few.
 template <class V>
 constexpr int dia_rpt() { return std::is_same<V, __half>::value ? 2 : 1; }
 
@@ -275,6 +278,7 @@ struct DevCsr {
     std::shared_ptr<DiaIndex> dia;     // diagonal index (shared by the copies), if built
     V *dval = nullptr, *rval = nullptr;  // this copy's diagonal and remainder values, if built
     bool use_dia = false;              // products through spmv_dia
+    bool dia2 = false;                 // ... in its second variant (FMT_DIA2)
     bool block_dia = false;            // block products (SpMM) through spmm_dia
     int nlev = 0;
     int *lrows = nullptr;
@@ -917,11 +921,14 @@ struct Krylov {
         if (m.use_dia && m.dval) {
             const DiaIndex &D = *m.dia;
             constexpr int RPT = dia_rpt<V>();
-            const int nb = (m.n + BS * RPT - 1) / (BS * RPT);
+            const int nb = m.dia2 ? (m.n + BS * 2 * RPT - 1) / (BS * 2 * RPT) : (m.n + BS * RPT - 1) / (BS * RPT);
             timed(cat, spmv_bytes<V, Epi>(m), [&] {
-                spmv_dia<V, T, Acc, Epi, 8, RPT><<<nb, BS, 0, stream>>>(m.n, D.ldn, D.nd, D.off, m.dval, D.rptr,
-                                                                        D.rcol, m.rval, m.rowscale, xin, epi, gate,
-                                                                        partials);
+                if (m.dia2)
+                    spmv_dia<V, T, Acc, Epi, 6, 2 * RPT><<<nb, BS, 0, stream>>>(
+                        m.n, D.ldn, D.nd, D.off, m.dval, D.rptr, D.rcol, m.rval, m.rowscale, xin, epi, gate, partials);
+                else
+                    spmv_dia<V, T, Acc, Epi, 8, RPT><<<nb, BS, 0, stream>>>(
+                        m.n, D.ldn, D.nd, D.off, m.dval, D.rptr, D.rcol, m.rval, m.rowscale, xin, epi, gate, partials);
             });
             if constexpr (Epi::K > 0)
                 timed(KC_FIN, 8.0 * Epi::K * nb,
@@ -1368,10 +1375,12 @@ static int set_format(DevCsr<V> &m, int f, hipStream_t s)
         m.slot16 = f == FMT_SLOT16;
         return CM_OK;
     case FMT_DIA:
+    case FMT_DIA2:
         if (!m.dia) return fail(CM_ERR_ARG, "diagonal format not available (no diagonal held by enough rows)");
         CM_TRY(build_dia_vals(m, s));
         m.use_slot = false;
         m.use_dia = true;
+        m.dia2 = f == FMT_DIA2;
         return CM_OK;
     default:
         return fail(CM_ERR_ARG, "unknown SpMV format");
@@ -1382,7 +1391,7 @@ template <class V>
 static int get_format(const DevCsr<V> &m)
 {
     if (!m.present()) return -1;
-    if (m.use_dia && m.dval) return FMT_DIA;
+    if (m.use_dia && m.dval) return m.dia2 ? FMT_DIA2 : FMT_DIA;
     if (m.use_slot && m.sval) return m.slot16 ? FMT_SLOT16 : FMT_SLOT;
     return (!std::is_same<V, double>::value && m.d16 && m.use16) ? FMT_STAGED16 : FMT_STAGED;
 }
@@ -1413,7 +1422,7 @@ static int tune_formats(KI &ki, int reps, hipEvent_t e0, hipEvent_t e1)
         if (m.dia) CM_TRY(ki.ensure_partials((m.n + BS - 1) / BS));  // at least the launch's workgroups
         float best = 1e30f;
         int pick = FMT_STAGED;
-        for (int f = FMT_STAGED; f <= FMT_DIA; f++) {
+        for (int f = FMT_STAGED; f <= FMT_DIA2; f++) {
             if (set_format(m, f, ki.stream) != CM_OK) continue;
             float ms = 0.f;
             CM_TRY(time_product(ki, m, reps, e0, e1, &ms));
@@ -2762,7 +2771,7 @@ int cm_system_spmv_format(cm_system h, int bits, int which, int format)
     DevCsr<double> *ms[] = {&k.A, &k.G, &k.H, &k.GT, &k.L, &k.U};
     if (which < 0 || which > 5) return fail(CM_ERR_ARG, "matrix index must be 0..5 (A, G, H, GT, L, U)");
     const bool need_slots = (format == FMT_SLOT || format == FMT_SLOT16) && !ms[which]->slots;
-    const bool need_dia = format == FMT_DIA && !ms[which]->dia;
+    const bool need_dia = (format == FMT_DIA || format == FMT_DIA2) && !ms[which]->dia;
     if (which < 4 && ms[which]->present() && (need_slots || need_dia)) {
         if (need_slots) CM_TRY(build_slots(*ms[which], k.stream));
         if (need_dia) CM_TRY(build_dia(*ms[which], k.stream));

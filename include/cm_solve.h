@@ -172,7 +172,9 @@ int cm_system_autotune(cm_system sys, int reps, int *tiles);
  *   3  slot-major blocks with 16-bit column offsets;
  *   4  diagonals held by at least 60% of the rows stored as dense n-vectors
  *      (no column indices), the other entries as a CSR remainder, one row
- *      per thread (matrices of structured grids).
+ *      per thread (matrices of structured grids; FP16 two rows per thread);
+ *   5  the same layout with twice the rows per thread and fewer diagonals in
+ *      flight (faster on matrices with many diagonals).
  * CM_ERR_ARG when the format is not available for that copy.  The
  * slot-major and diagonal indexes are built on first use (and shared by the
  * precisions). */

@@ -269,11 +269,15 @@ __global__ __launch_bounds__(BS) void spmm_short(int cnt, const int *__restrict_
 // value is one load per row (shared by its lanes), its x segment the row's
 // KB columns at row i + off[d], so consecutive rows read consecutive x
 // segments and no column index is read for the diagonal part.  Row sums in
-// diagonal order, then the remainder.
+// diagonal order, then the remainder.  Rows with long remainders (lpos[row]
+// >= 0; lpos null when there are none) take their remainder sums from racc
+// (KB values per long row, written by a wave-per-row pass before this one)
+// instead of walking the remainder themselves.
 template <class V, class T, class Acc, int KB, class Epi>
 __global__ __launch_bounds__(BS) void spmm_dia(int n, int ldn, int nd, const int *__restrict__ off,
                                                const V *__restrict__ dval, const int *__restrict__ rptr,
                                                const int *__restrict__ rcol, const V *__restrict__ rval,
+                                               const int *__restrict__ lpos, const T *__restrict__ racc,
                                                const float *__restrict__ rowscale, const T *__restrict__ x, Epi epi,
                                                const BlockFlags *fl, double *partials, int slot0)
 {
@@ -317,7 +321,13 @@ __global__ __launch_bounds__(BS) void spmm_dia(int n, int ldn, int nd, const int
                     for (int i = 0; i < CPL; i++) s[i] += v[q] * (Acc)xv[q][i];
                 }
         }
-        if (rptr)
+        const int lq = lpos ? lpos[r] : -1;
+        if (lq >= 0) {
+            T xr[CPL];
+            ld_cols<T, CPL>(racc + (size_t)lq * KB + c0, xr);
+#pragma unroll
+            for (int i = 0; i < CPL; i++) s[i] += (Acc)xr[i];
+        } else if (rptr) {
             for (int k = rptr[r]; k < rptr[r + 1]; k++) {
                 T xr[CPL];
                 ld_cols<T, CPL>(x + (size_t)rcol[k] * KB + c0, xr);
@@ -325,6 +335,7 @@ __global__ __launch_bounds__(BS) void spmm_dia(int n, int ldn, int nd, const int
 #pragma unroll
                 for (int i = 0; i < CPL; i++) s[i] += vk * (Acc)xr[i];
             }
+        }
         const Acc sc = rowscale ? (Acc)rowscale[r] : (Acc)1;
 #pragma unroll
         for (int i = 0; i < CPL; i++) epi.row((size_t)r * KB + c0 + i, c0 + i, rowscale ? s[i] * sc : s[i], acc[i]);

@@ -224,9 +224,18 @@ struct DiaIndex {
     int n = 0, ldn = 0, nd = 0;  // ldn: diagonal stride, n rounded up to DIA_ALIGN
     long long nrem = 0;
     int *off = nullptr, *dsrc = nullptr, *rptr = nullptr, *rcol = nullptr, *rsrc = nullptr;
+    // Rows whose remainder is longer than SHORT_MAX, for the block products:
+    // lpos[row] = their index in a compact CSR of those remainders (lptr /
+    // lcol, values gathered through lsrc), -1 for the other rows; lrows =
+    // 0..nlong-1 (the row list of the wave-per-row kernel).
+    int nlong = 0;
+    long long nlrem = 0;
+    int *lpos = nullptr, *lptr = nullptr, *lcol = nullptr, *lsrc = nullptr, *lrows = nullptr;
     size_t bytes() const
     {
-        return ((size_t)nd + (size_t)nd * ldn + (rptr ? (size_t)n + 1 : 0) + 2 * (size_t)nrem) * sizeof(int);
+        return ((size_t)nd + (size_t)nd * ldn + (rptr ? (size_t)n + 1 : 0) + 2 * (size_t)nrem +
+                (nlong ? (size_t)n + 2 * (size_t)nlong + 1 + 2 * (size_t)nlrem : 0)) *
+               sizeof(int);
     }
     ~DiaIndex()
     {
@@ -235,6 +244,11 @@ struct DiaIndex {
         (void)hipFree(rptr);
         (void)hipFree(rcol);
         (void)hipFree(rsrc);
+        (void)hipFree(lpos);
+        (void)hipFree(lptr);
+        (void)hipFree(lcol);
+        (void)hipFree(lsrc);
+        (void)hipFree(lrows);
     }
 };
 
@@ -277,6 +291,7 @@ struct DevCsr {
     bool slot16 = false;               // ... with its 16-bit column offsets
     std::shared_ptr<DiaIndex> dia;     // diagonal index (shared by the copies), if built
     V *dval = nullptr, *rval = nullptr;  // this copy's diagonal and remainder values, if built
+    V *lval = nullptr;                   // ... and its long-row remainder values (block products)
     bool use_dia = false;              // products through spmv_dia
     bool dia2 = false;                 // ... in its second variant (FMT_DIA2)
     bool block_dia = false;            // block products (SpMM) through spmm_dia
@@ -290,7 +305,7 @@ struct DevCsr {
         if (!present()) return 0;
         size_t b = (size_t)nnz * sizeof(V) + (rowscale ? (size_t)n * sizeof(float) : 0);
         if (sval) b += (size_t)nnz * sizeof(V);
-        if (dval && dia) b += ((size_t)dia->nd * dia->ldn + (size_t)dia->nrem) * sizeof(V);
+        if (dval && dia) b += ((size_t)dia->nd * dia->ldn + (size_t)dia->nrem + (size_t)dia->nlrem) * sizeof(V);
         if (owns_index) {
             if (slots) b += slots->bytes();
             if (dia) b += dia->bytes();
@@ -314,9 +329,10 @@ struct DevCsr {
         (void)hipFree(sval);
         (void)hipFree(dval);
         (void)hipFree(rval);
+        (void)hipFree(lval);
         ptr = col = blk = lrows = cbase = nullptr;
         d16 = nullptr;
-        val = sval = dval = rval = nullptr;
+        val = sval = dval = rval = lval = nullptr;
         rowscale = nullptr;
         slots.reset();
         dia.reset();
@@ -556,11 +572,35 @@ static int build_dia(DevCsr<double> &m, hipStream_t s)
         }
         rptr[i + 1] = (int)rcol.size();
     }
+    // long remainders (block products): a compact CSR of the rows with more
+    // than SHORT_MAX remainder entries
+    std::vector<int> lpos(n, -1), lptr{0}, lcol, lsrc;
+    for (int i = 0; i < n; i++) {
+        if (rptr[i + 1] - rptr[i] <= SHORT_MAX) continue;
+        lpos[i] = (int)lptr.size() - 1;
+        for (int k = rptr[i]; k < rptr[i + 1]; k++) {
+            lcol.push_back(rcol[k]);
+            lsrc.push_back(rsrc[k]);
+        }
+        lptr.push_back((int)lcol.size());
+    }
     auto D = std::make_shared<DiaIndex>();
     D->n = n;
     D->ldn = ldn;
     D->nd = nd;
     D->nrem = (long long)rcol.size();
+    D->nlong = (int)lptr.size() - 1;
+    D->nlrem = (long long)lcol.size();
+    if (D->nlong) {
+        std::vector<int> iota(D->nlong);
+        for (int q = 0; q < D->nlong; q++) iota[q] = q;
+        CM_TRY(dev_upload(&D->lpos, lpos.data(), lpos.size(), s));
+        CM_TRY(dev_upload(&D->lptr, lptr.data(), lptr.size(), s));
+        CM_TRY(dev_upload(&D->lcol, lcol.data(), lcol.size(), s));
+        CM_TRY(dev_upload(&D->lsrc, lsrc.data(), lsrc.size(), s));
+        CM_TRY(dev_upload(&D->lrows, iota.data(), iota.size(), s));
+        CM_HIP(hipStreamSynchronize(s));  // host temporaries
+    }
     CM_TRY(dev_upload(&D->off, off.data(), off.size(), s));
     CM_TRY(dev_upload(&D->dsrc, dsrc.data(), dsrc.size(), s));
     if (D->nrem) {
@@ -584,6 +624,10 @@ static int build_dia_vals(DevCsr<V> &m, hipStream_t s)
     CM_TRY(dev_alloc(&m.rval, (size_t)std::max<long long>(D.nrem, 1)));
     k_gather_vals<V><<<long_grid(nde), BS, 0, s>>>(nde, D.dsrc, m.val, m.dval);
     if (D.nrem) k_gather_vals<V><<<long_grid(D.nrem), BS, 0, s>>>(D.nrem, D.rsrc, m.val, m.rval);
+    if (D.nlrem) {
+        CM_TRY(dev_alloc(&m.lval, (size_t)D.nlrem));
+        k_gather_vals<V><<<long_grid(D.nlrem), BS, 0, s>>>(D.nlrem, D.lsrc, m.val, m.lval);
+    }
     CM_HIP(hipGetLastError());
     return CM_OK;
 }
@@ -1337,7 +1381,8 @@ static void follow_blocks(DevCsr<V> &dst, const DevCsr<double> &src)
     if (dst.dia != src.dia) {
         (void)hipFree(dst.dval);
         (void)hipFree(dst.rval);
-        dst.dval = dst.rval = nullptr;
+        (void)hipFree(dst.lval);
+        dst.dval = dst.rval = dst.lval = nullptr;
         dst.dia = src.dia;
         dst.use_dia = dst.block_dia = false;
     }
@@ -1500,6 +1545,8 @@ struct BlockKrylov {
     BlockFlags *fl = nullptr, *hfl = nullptr;  // device flags; pinned host copies (3)
     double *partials = nullptr, *partials2 = nullptr;  // slot partials; first-stage sums
     size_t partial_slots = 0;
+    T *racc = nullptr;  // long-row remainder sums of the diagonal block products (KB per long row)
+    size_t racc_cap = 0;
     hipEvent_t ev[2] = {nullptr, nullptr};
     bool use_graph = true;
     hipGraphExec_t g1 = nullptr, gN = nullptr;  // one pass, GRAPH_PASSES passes
@@ -1509,7 +1556,7 @@ struct BlockKrylov {
         if (stream) (void)hipStreamSynchronize(stream);
         if (g1) (void)hipGraphExecDestroy(g1);
         if (gN) (void)hipGraphExecDestroy(gN);
-        T *bufs[] = {b, x, r0, r, p, v, s, t, pc, sc, tm};
+        T *bufs[] = {b, x, r0, r, p, v, s, t, pc, sc, tm, racc};
         for (T *q : bufs) (void)hipFree(q);
         (void)hipFree(st);
         (void)hipFree(fl);
@@ -1549,6 +1596,27 @@ struct BlockKrylov {
     static int long_grid(const RowBins &bn) { return std::min((bn.nl + (BS / WAVE) - 1) / (BS / WAVE), SPMM_CAP); }
     static int dia_grid(int nrows) { return std::min((nrows + SROWS - 1) / SROWS, SPMM_CAP); }
 
+    // Room for the long-row remainder sums of the matrices that use the
+    // diagonal block products (called outside graph capture, before a solve).
+    int ensure_racc()
+    {
+        size_t need = 0;
+        auto one = [&](const auto *m) {
+            if (m && m->block_dia && m->dia) need = std::max(need, (size_t)m->dia->nlong * KB);
+        };
+        one(A);
+        one(G);
+        one(Hm);
+        if (need <= racc_cap) return CM_OK;
+        CM_HIP(hipStreamSynchronize(stream));
+        drop_graphs();  // their launches carry the old buffer
+        (void)hipFree(racc);
+        racc = nullptr;
+        CM_HIP(hipMalloc((void **)&racc, need * sizeof(T)));
+        racc_cap = need;
+        return CM_OK;
+    }
+
     // device bytes of a block solver of this type for an n-row system whose
     // three matrices have (short + long) row-bin workgroups `slots`
     static size_t bytes_for(int n, size_t slots)
@@ -1587,6 +1655,7 @@ struct BlockKrylov {
         CM_HIP(hipMalloc((void **)&partials, partial_slots * 4 * KB * sizeof(double)));
         CM_HIP(hipMalloc((void **)&partials2, (partial_slots / FIN_PER + 1) * 4 * KB * sizeof(double)));
         for (auto &e : ev) CM_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+        CM_TRY(ensure_racc());
         CM_HIP(hipStreamSynchronize(stream));
         return CM_OK;
     }
@@ -1601,8 +1670,14 @@ struct BlockKrylov {
         if (m.block_dia && m.dval) {
             const DiaIndex &D = *m.dia;
             const int gd = dia_grid(n);
+            if (D.nlong) {  // long remainders first, a wave per row, sums into racc
+                const int gl = std::min((D.nlong + (BS / WAVE) - 1) / (BS / WAVE), SPMM_CAP);
+                spmm_long<V, T, Acc, KB, BEpiStore<T>><<<gl, BS, 0, stream>>>(
+                    D.nlong, D.lrows, D.lptr, D.lcol, m.lval, nullptr, xin, BEpiStore<T>{racc}, gf, partials, 0);
+            }
             spmm_dia<V, T, Acc, KB, Epi><<<gd, BS, 0, stream>>>(n, D.ldn, D.nd, D.off, m.dval, D.rptr, D.rcol, m.rval,
-                                                               m.rowscale, xin, epi, gf, partials, 0);
+                                                               D.nlong ? D.lpos : nullptr, racc, m.rowscale, xin,
+                                                               epi, gf, partials, 0);
             if constexpr (Epi::K > 0) finalize<Epi::K>(gd, epi, fst, gate, gate ? 1 : 0);
             return;
         }
@@ -1640,7 +1715,8 @@ struct BlockKrylov {
         if (m.block_dia && m.dval) {  // diagonals (zeros included) and remainder once, X and Y once
             const DiaIndex &D = *m.dia;
             *bytes = (double)D.nd * m.n * sizeof(V) + 4.0 * D.nd + (double)D.nrem * (sizeof(V) + sizeof(int)) +
-                     (D.rptr ? 4.0 * (m.n + 1) : 0.0) + 2.0 * (double)ne() * sizeof(T) + (m.rowscale ? 4.0 * m.n : 0.0);
+                     (D.rptr ? 4.0 * (m.n + 1) : 0.0) + (D.nlong ? 4.0 * m.n + 8.0 * D.nlong : 0.0) +
+                     2.0 * (double)ne() * sizeof(T) + (m.rowscale ? 4.0 * m.n : 0.0);
         } else {
             *bytes = (double)m.nnz * (sizeof(V) + sizeof(int)) + 4.0 * (m.n + 1) + 4.0 * (bn.ns + bn.nl) +
                      2.0 * (double)ne() * sizeof(T) + (m.rowscale ? 4.0 * m.n : 0.0);
@@ -1930,6 +2006,7 @@ struct BlockSolver : BlockBase {
         if (!ki) return fail(CM_ERR_ARG, "mixed precision not enabled (cm_system_enable_mixed)");
         auto q = std::make_unique<BI>();
         CM_TRY(q->create(*ki, stream));
+        CM_TRY(q->ensure_racc());
         q->use_graph = graphs;
         bi = std::move(q);
         return CM_OK;
@@ -1939,6 +2016,9 @@ struct BlockSolver : BlockBase {
     {
         ncols = nc;
         b64.ncols = nc;
+        CM_TRY(b64.ensure_racc());
+        if (b32) CM_TRY(b32->ensure_racc());
+        if (b16) CM_TRY(b16->ensure_racc());
         const int g = (int)std::min<size_t>(((size_t)n * KB + BS - 1) / BS, 8192);
         if (B) {  // B null: the resident right-hand sides are kept, only x is reset
             CM_HIP(hipMemcpyAsync(cm, B, (size_t)n * nc * sizeof(double), hipMemcpyHostToDevice, stream));
@@ -2158,7 +2238,8 @@ static int make_block(std::unique_ptr<BlockBase> (&cache)[5], BlockBase *&cur, i
 
 // Block products in the diagonal format: a matrix qualifies when its
 // diagonal index exists and leaves at most BLOCK_DIA_REM of the nonzeros to
-// the CSR remainder (long irregular rows stay with the row-binned kernels).
+// the CSR remainder outside the long rows (remainders over SHORT_MAX entries,
+// which a wave-per-row pass sums first).
 constexpr double BLOCK_DIA_REM = 0.3;
 
 // Results of the last mixed-precision solve.
@@ -2219,7 +2300,8 @@ static int prepare_block_dia(System &S)
         bool ok = false;
         if (S.block_dia) {
             CM_TRY(build_dia(m, k.stream));
-            ok = m.dia && (double)m.dia->nrem <= BLOCK_DIA_REM * (double)m.nnz;
+            // the long remainders go to the wave-per-row kernel: the limit is on the others
+            ok = m.dia && (double)(m.dia->nrem - m.dia->nlrem) <= BLOCK_DIA_REM * (double)m.nnz;
             if (ok) CM_TRY(build_dia_vals(m, k.stream));
         }
         m.block_dia = ok;

@@ -60,13 +60,18 @@ def test_block_afiim_columns(k):
     _check_columns("afiim", A, F, B, X, rv, it, info, 500)
 
 
-@pytest.mark.parametrize("k", [1, 5, 16])
-def test_block_dia_matches_row_binned(k):
+@pytest.mark.parametrize("kind,k", [("cd3", 1), ("cd3", 5), ("cd3", 16), ("long_rows", 5), ("long_rows", 16)])
+def test_block_dia_matches_row_binned(kind, k):
     """Block products in the diagonal format (the default for stencil and
-    factor matrices) against the row-binned kernels (block_dia off): both meet
-    the oracle checks in FP64, both FP32- and FP16-inner refinements reach
-    the tolerance, and the two agree to 1e-6."""
-    A = cmamd.gen_convdiff(3, 20, 2.0, seed=17)
+    factor matrices; with long rows, their remainders summed by a wave-per-row
+    pass first) against the row-binned kernels (block_dia off): both meet the
+    oracle checks in FP64, both FP32- and FP16-inner refinements reach the
+    tolerance, and the two agree to 1e-6."""
+    if kind == "cd3":
+        A = cmamd.gen_convdiff(3, 20, 2.0, seed=17)
+    else:  # 27-point stencil, 5% of rows with 200-600 long-range couplings
+        A = cmamd.gen_stencil27(14, 0.05, 200, 600, seed=9)
+        assert (np.diff(A.ptr) > 64).sum() > 0
     F = cmamd.afiim(A, 0.1)
     B = _rhs(A, k, 70)
     out = {}

@@ -488,7 +488,12 @@ def bench_serve(args, w):
         st[key] -= st0[key]
     srv.close()
     # plain FP64 one-at-a-time: each column a single solve on its resident
-    # system, in arrival order, one at a time
+    # system, in arrival order, one at a time, on autotuned single-RHS SpMV formats
+    t_tune = time.perf_counter()
+    if not args.no_autotune:
+        for h in handles:
+            h.autotune(reps=3)
+    t_tune = time.perf_counter() - t_tune
     lock = threading.Lock()
     queue = []
     cv = threading.Condition(lock)
@@ -571,7 +576,9 @@ def bench_serve(args, w):
         "rel_error_vs_fp64_solution_max": worst_64, "fp64_comparison_sample": f"every {SAMPLE}th request",
         "fp64_one_at_a_time": {"value": round(v64, 3), "unit": "solves/s",
                                "latency_ms": {"p50": _pct(lat_b, 50), "p99": _pct(lat_b, 99)},
-                               "wall_s": round(t_base, 2), "requests": nb},
+                               "wall_s": round(t_base, 2), "requests": nb,
+                               "spmv": "autotuned single-RHS formats" if not args.no_autotune else "staged",
+                               "autotune_s": round(t_tune, 1)},
         "speedup_vs_fp64_one_at_a_time": round(value / v64, 3),
         "wall_s": round(t_served, 2),
         "roofline": None, "cpu_baseline": None,

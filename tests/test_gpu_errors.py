@@ -72,3 +72,52 @@ def test_server_rejects():
     assert r.info[0] == 0
     srv.close()
     s.close()
+
+
+def test_spmv_format_errors():
+    """Format selection refuses what a copy does not have, with CM_ERR_ARG,
+    and leaves the current format in place; the raw C entry point refuses an
+    unknown format number and a matrix index out of range."""
+    A, F = _small()
+    s = cmamd.System("afiim", A, F)
+    before = s.spmv_formats()
+    for bits, which, fmt in ((64, "L", "dia"),        # AFIIM has no L factor
+                             (64, "A", "staged16"),   # no 16-bit staged form in FP64
+                             (32, "A", "dia"),        # mixed precision not enabled
+                             (16, "G", "slot")):
+        with pytest.raises(cmamd.EngineError) as e:
+            s.spmv_format(bits, which, fmt)
+        assert e.value.code == -2
+    assert s.spmv_formats() == before
+    lib = cmamd.solve_lib()
+    assert lib.cm_system_spmv_format(s._h, 64, 0, 9) == -2
+    assert lib.cm_system_spmv_format(s._h, 64, 7, 0) == -2
+    assert lib.cm_system_spmv_format(s._h, 48, 0, 0) == -2
+    # still solves, in a format that exists
+    s.spmv_format(64, "A", "dia")
+    s.set_rhs(cmamd.csr_mv_host(A, np.ones(A.n)))
+    s.set_x(None)
+    assert s.solve(tol=1e-8, nmax=200)[2] == 0
+    s.close()
+
+
+def test_block_resident_and_dia_switch_errors():
+    """block_set without right-hand sides needs a resident block of the same
+    width; switching the diagonal block products off and on keeps solves
+    right."""
+    A, F = _small()
+    s = cmamd.System("afiim", A, F)
+    lib = cmamd.solve_lib()
+    assert lib.cm_system_block_set(s._h, 3, None, None) == -2  # nothing resident yet
+    B = np.stack([cmamd.csr_mv_host(A, cmamd.gen_uniform(80 + j, A.n)) for j in range(3)], axis=1)
+    s.block_set(B)
+    assert lib.cm_system_block_set(s._h, 2, None, None) == -2  # other width
+    for on in (False, True, False):
+        s.block_dia(on)
+        s.block_set(None)
+        rv, it, info = s.block_solve(tol=1e-8, nmax=300)
+        assert np.all(info == 0)
+        X = s.block_get()
+        for j in range(3):
+            assert np.linalg.norm(B[:, j] - cmamd.csr_mv_host(A, X[:, j])) / np.linalg.norm(B[:, j]) <= 1e-6
+    s.close()

@@ -277,6 +277,7 @@ SOLVE_SYMBOLS = [
     "cm_afiim_epbicgstab", "cm_pdfiim_epbicgstab", "cm_acifim_pbicgstab", "cm_system_create",
     "cm_system_destroy", "cm_system_set_rhs", "cm_system_set_x", "cm_system_get_x", "cm_system_solve",
     "cm_system_spmv", "cm_system_apply", "cm_system_nnz", "cm_system_profile", "cm_system_kernel_stats",
+    "cm_system_kernel_format_bytes",
     "cm_system_enable_mixed", "cm_system_solve_mixed",
     "cm_system_block_set", "cm_system_block_get", "cm_system_block_solve", "cm_system_block_solve_mixed",
     "cm_system_graphs", "cm_system_block_graphs", "cm_system_block_release",
@@ -361,6 +362,8 @@ def _type_solve(lib):
     lib.cm_system_block_dia.restype = C.c_int
     lib.cm_system_kernel_stats.argtypes = [vp, C.c_int, C.POINTER(C.c_longlong), PD, PD]
     lib.cm_system_kernel_stats.restype = C.c_int
+    lib.cm_system_kernel_format_bytes.argtypes = [vp, C.c_int, PD]
+    lib.cm_system_kernel_format_bytes.restype = C.c_int
     for f in ("cm_system_destroy", "cm_system_set_rhs", "cm_system_set_x", "cm_system_get_x",
               "cm_system_solve", "cm_system_spmv", "cm_system_apply"):
         getattr(lib, f).restype = C.c_int
@@ -587,12 +590,24 @@ class System:
         _check(self._lib.cm_system_profile_sampled(self._h, mask, int(sample)))
 
     def kernel_stats(self):
-        """{category: (launches, total_ms, total_bytes)} for the profiled solves."""
+        """{category: (launches, total_ms, total_bytes)} for the profiled solves;
+        total_bytes is the format-independent algorithmic traffic (CSR figure
+        for the SpMV categories)."""
         out = {}
         for c, name in enumerate(self.KERNEL_CATS):
             n, ms, by = C.c_longlong(), C.c_double(), C.c_double()
             _check(self._lib.cm_system_kernel_stats(self._h, c, C.byref(n), C.byref(ms), C.byref(by)))
             out[name] = (n.value, ms.value, by.value)
+        return out
+
+    def kernel_format_bytes(self):
+        """{category: bytes the counted launches touch once in the storage
+        format they ran in} for the profiled solves."""
+        out = {}
+        for c, name in enumerate(self.KERNEL_CATS):
+            by = C.c_double()
+            _check(self._lib.cm_system_kernel_format_bytes(self._h, c, C.byref(by)))
+            out[name] = by.value
         return out
 
 

@@ -744,13 +744,13 @@ struct Prof {
     struct Rec {
         int cat, pass;
         bool p_update;
-        double bytes;
+        double bytes, fbytes;
         hipEvent_t e0, e1;
     };
     std::vector<Rec> recs;
     std::vector<hipEvent_t> pool;
     long long launches[KC_N] = {};
-    double ms[KC_N] = {}, bytes[KC_N] = {};
+    double ms[KC_N] = {}, bytes[KC_N] = {}, fbytes[KC_N] = {};
 
     hipEvent_t get()
     {
@@ -769,6 +769,7 @@ struct Prof {
             launches[c] = 0;
             ms[c] = 0.0;
             bytes[c] = 0.0;
+            fbytes[c] = 0.0;
         }
     }
     // Fold the finished solve's records into the totals.  A pass executes
@@ -784,6 +785,7 @@ struct Prof {
                     launches[r.cat]++;
                     ms[r.cat] += t;
                     bytes[r.cat] += r.bytes;
+                    fbytes[r.cat] += r.fbytes;
                 }
             }
             if (r.e0) pool.push_back(r.e0);
@@ -917,23 +919,37 @@ struct Krylov {
     }
 
     // Launch f on the stream, bracketed by timing events when profiling.
+    // bytes: the launch's algorithmic traffic; fbytes: what the storage
+    // format in use touches once (the same unless a format reads more or
+    // less than plain CSR; < 0 means the same).
     template <class F>
-    void timed(int cat, double bytes, F f)
+    void timed(int cat, double bytes, F f, double fbytes = -1.0)
     {
         if (!prof.on || !(prof.mask & (1u << cat))) {
             f();
             return;
         }
-        Prof::Rec rec{cat, prof.pass, prof.p_update, bytes, prof.get(), prof.get()};
+        Prof::Rec rec{cat, prof.pass, prof.p_update, bytes, fbytes < 0.0 ? bytes : fbytes, prof.get(), prof.get()};
         if (rec.e0) (void)hipEventRecord(rec.e0, stream);
         f();
         if (rec.e1) (void)hipEventRecord(rec.e1, stream);
         prof.recs.push_back(rec);
     }
 
-    // Compulsory bytes of one SpMV launch: values + column indices once, row
-    // pointers, row blocks and row scales once, x once, y written once, plus
-    // the epilogue's extra n-vectors.
+    // Algorithmic bytes of one SpMV launch, independent of the storage
+    // format: the CSR matrix once (values, column indices, row pointers, and
+    // the per-row scales of FP16 values), x once, y written once, plus the
+    // epilogue's extra n-vectors.
+    template <class V, class Epi>
+    double alg_bytes(const DevCsr<V> &m) const
+    {
+        return (double)m.nnz * (sizeof(V) + 4.0) + 4.0 * (m.n + 1) + (m.rowscale ? 4.0 * m.n : 0.0) +
+               (double)sizeof(T) * m.n * (2 + Epi::XV);
+    }
+
+    // Compulsory bytes of one SpMV launch in the format in use: its matrix
+    // arrays once (row blocks, slot tables or diagonals with their zeros
+    // included), x once, y written once, plus the epilogue's extra n-vectors.
     template <class V, class Epi>
     double spmv_bytes(const DevCsr<V> &m) const
     {
@@ -966,14 +982,14 @@ struct Krylov {
             const DiaIndex &D = *m.dia;
             constexpr int RPT = dia_rpt<V>();
             const int nb = m.dia2 ? (m.n + BS * 2 * RPT - 1) / (BS * 2 * RPT) : (m.n + BS * RPT - 1) / (BS * RPT);
-            timed(cat, spmv_bytes<V, Epi>(m), [&] {
+            timed(cat, alg_bytes<V, Epi>(m), [&] {
                 if (m.dia2)
                     spmv_dia<V, T, Acc, Epi, 6, 2 * RPT><<<nb, BS, 0, stream>>>(
                         m.n, D.ldn, D.nd, D.off, m.dval, D.rptr, D.rcol, m.rval, m.rowscale, xin, epi, gate, partials);
                 else
                     spmv_dia<V, T, Acc, Epi, 8, RPT><<<nb, BS, 0, stream>>>(
                         m.n, D.ldn, D.nd, D.off, m.dval, D.rptr, D.rcol, m.rval, m.rowscale, xin, epi, gate, partials);
-            });
+            }, spmv_bytes<V, Epi>(m));
             if constexpr (Epi::K > 0)
                 timed(KC_FIN, 8.0 * Epi::K * nb,
                       [&] { k_finalize<Epi::K, Epi><<<1, FBS, 0, stream>>>(nb, partials, kst, epi); });
@@ -981,20 +997,20 @@ struct Krylov {
         }
         if (m.use_slot && m.sval) {
             const SlotIndex &S = *m.slots;
-            timed(cat, spmv_bytes<V, Epi>(m), [&] {
+            timed(cat, alg_bytes<V, Epi>(m), [&] {
                 if (m.slot16)
                     spmv_slot<V, T, Acc, Epi, true><<<S.nb, BS, 0, stream>>>(
                         S.hdr, S.soff, S.lrow, S.col, S.d16, S.cbase, m.sval, m.rowscale, xin, epi, gate, partials);
                 else
                     spmv_slot<V, T, Acc, Epi, false><<<S.nb, BS, 0, stream>>>(
                         S.hdr, S.soff, S.lrow, S.col, S.d16, S.cbase, m.sval, m.rowscale, xin, epi, gate, partials);
-            });
+            }, spmv_bytes<V, Epi>(m));
             if constexpr (Epi::K > 0)
                 timed(KC_FIN, 8.0 * Epi::K * S.nb,
                       [&] { k_finalize<Epi::K, Epi><<<1, FBS, 0, stream>>>(S.nb, partials, kst, epi); });
             return;
         }
-        timed(cat, spmv_bytes<V, Epi>(m), [&] {
+        timed(cat, alg_bytes<V, Epi>(m), [&] {
             if constexpr (!std::is_same<V, double>::value) {
                 if (m.d16 && m.use16) {  // low precision with 16-bit indices
                     if (m.tile == 1024)
@@ -1031,7 +1047,7 @@ struct Krylov {
             else
                 spmv_kernel<2048, V, T, Acc, Epi><<<m.nblk, BS, 0, stream>>>(m.ptr, m.col, m.val, m.rowscale,
                                                                             m.blk, xin, epi, gate, partials);
-        });
+        }, spmv_bytes<V, Epi>(m));
         if constexpr (Epi::K > 0)
             timed(KC_FIN, 8.0 * Epi::K * m.nblk,
                   [&] { k_finalize<Epi::K, Epi><<<1, FBS, 0, stream>>>(m.nblk, partials, kst, epi); });
@@ -3050,6 +3066,18 @@ int cm_system_kernel_stats(cm_system h, int category, long long *launches, doubl
     if (launches) *launches = nl;
     if (total_ms) *total_ms = ms;
     if (total_bytes) *total_bytes = by;
+    return CM_OK;
+}
+
+int cm_system_kernel_format_bytes(cm_system h, int category, double *total_bytes)
+{
+    if (!h || category < 0 || category >= KC_N || !total_bytes) return fail(CM_ERR_ARG, "bad category or null output");
+    System &S = h->sys;
+    double by = 0.0;
+    const Prof *ps[] = {&S.k64->prof, S.k32 ? &S.k32->prof : nullptr, S.k16 ? &S.k16->prof : nullptr};
+    for (const Prof *p : ps)
+        if (p) by += p->fbytes[category];
+    *total_bytes = by;
     return CM_OK;
 }
 

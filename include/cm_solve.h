@@ -210,7 +210,9 @@ int cm_system_apply(cm_system sys, const double *v, double *out);
  * their fused dots), 1 first factor product (H, or G^T, with D^{-1}), 2 G
  * products, 3 L/U products (mode C), 4 vector updates, 5 setup, 6 the
  * one-workgroup reduction finalizers.  total_bytes
- * is the algorithmic (compulsory) traffic of the counted launches. */
+ * is the algorithmic traffic of the counted launches: for the SpMV categories
+ * the CSR figure nnz*(value bytes + 4) + 4*(n+1) (+ 4*n row scales for FP16
+ * values) plus the vectors read and written, whatever storage format runs. */
 int cm_system_profile(cm_system sys, int mask);
 /* As cm_system_profile, but only the first `passes` passes of each solve
  * are bracketed (and launched kernel by kernel); the other passes replay
@@ -219,6 +221,12 @@ int cm_system_profile(cm_system sys, int mask);
 int cm_system_profile_sampled(cm_system sys, int mask, int passes);
 int cm_system_kernel_stats(cm_system sys, int category, long long *launches, double *total_ms,
                            double *total_bytes);
+/* The bytes the same counted launches touch once in the storage format each
+ * one ran in (row-block tables, slot tables, or the diagonals with their
+ * zero fill and no column indices; equal to total_bytes' vector terms plus
+ * the format's matrix arrays).  For the non-SpMV categories it equals
+ * total_bytes. */
+int cm_system_kernel_format_bytes(cm_system sys, int category, double *total_bytes);
 
 /* Stored sizes (after symmetric expansion / transposition): which as above. */
 long long cm_system_nnz(cm_system sys, char which);

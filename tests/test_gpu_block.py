@@ -60,7 +60,8 @@ def test_block_afiim_columns(k):
     _check_columns("afiim", A, F, B, X, rv, it, info, 500)
 
 
-@pytest.mark.parametrize("kind,k", [("cd3", 1), ("cd3", 5), ("cd3", 16), ("long_rows", 5), ("long_rows", 16)])
+@pytest.mark.parametrize("kind,k", [("cd3", 1), ("cd3", 5), ("cd3", 16), ("long_rows", 5), ("long_rows", 16),
+                                    ("long_rows_large", 5), ("long_rows_large", 16)])
 def test_block_dia_matches_row_binned(kind, k):
     """Block products in the diagonal format (the default for stencil and
     factor matrices; with long rows, their remainders summed by a wave-per-row
@@ -69,8 +70,11 @@ def test_block_dia_matches_row_binned(kind, k):
     tolerance, and the two agree to 1e-6."""
     if kind == "cd3":
         A = cmamd.gen_convdiff(3, 20, 2.0, seed=17)
-    else:  # 27-point stencil, 5% of rows with 200-600 long-range couplings
+    elif kind == "long_rows":  # 27-point stencil, 5% of rows with 200-600 long-range couplings
         A = cmamd.gen_stencil27(14, 0.05, 200, 600, seed=9)
+        assert (np.diff(A.ptr) > 64).sum() > 0
+    else:  # the same at 52^3 rows (about 7000 long rows over 140608 columns)
+        A = cmamd.gen_stencil27(52, 0.05, 200, 600, seed=9)
         assert (np.diff(A.ptr) > 64).sum() > 0
     F = cmamd.afiim(A, 0.1)
     B = _rhs(A, k, 70)

@@ -394,6 +394,36 @@ def test_sampled_profiling_counts_and_results():
     sys_.close()
 
 
+def test_algorithmic_bytes_do_not_depend_on_format():
+    """The per-launch algorithmic bytes of the H products (with D^{-1}) are
+    the CSR figure nnz(H)*(8 + 4) + 4*(n+1) + 3 FP64 n-vectors in every
+    storage format; the format's own bytes differ from it (row-block table
+    added for the staged kernel, diagonal zeros in and diagonal column
+    indices out for the diagonal one)."""
+    A = _system("cd2d")
+    b = cmamd.csr_mv_host(A, cmamd.gen_uniform(2, A.n))
+    F = cmamd.afiim(A, 0.1)
+    sys_ = cmamd.System("afiim", A, F)
+    sys_.set_rhs(b)
+    n, nnz_h = A.n, F.H.nnz
+    alg = nnz_h * 12.0 + 4.0 * (n + 1) + 3 * 8.0 * n
+    seen = {}
+    for fmt in ("staged", "dia", "slot"):
+        sys_.spmv_format(64, "H", fmt)
+        sys_.profile(True, cats=("F1",))
+        sys_.set_x(None)
+        sys_.solve(tol=1e-8, nmax=500)
+        nl, _, by = sys_.kernel_stats()["F1"]
+        fb = sys_.kernel_format_bytes()["F1"]
+        assert nl > 0
+        assert by / nl == pytest.approx(alg, rel=1e-12)
+        seen[fmt] = fb / nl
+        assert seen[fmt] != pytest.approx(alg, rel=1e-9)
+    assert seen["staged"] > alg  # CSR plus the row-block table
+    sys_.profile(False)
+    sys_.close()
+
+
 def test_graph_replays_match_eager_single():
     """Single-RHS passes replayed from captured hipGraphs against kernel-by-
     kernel launches: bit-identical traces and solutions over several replays,

@@ -78,7 +78,8 @@ PMC_CATEGORIES = (("EpiScale<double", "F1"), ("EpiStore<double", "F2"), ("EpiAlp
 def pmc_summary(fetch_db, write_db, copy_fetch_db, copy_write_db, bench_json, workload):
     """HBM bytes per dispatch from two rocprofv3 --pmc runs of the bench (FETCH_SIZE, WRITE_SIZE, KiB
     per dispatch) scaled by what the same counters read for a plain 1 GiB device copy, per kernel of
-    the FP64 solve; next to the compulsory bytes the bench's own table gives.  Writes
+    the FP64 solve; next to the bytes the storage format in use touches once (compulsory) and the
+    CSR algorithmic bytes, both from the bench's own table.  Writes
     profiles/traffic_<workload>.json (read by the bench for roofline.traffic) and prints the table."""
     import sqlite3
 
@@ -102,7 +103,9 @@ def pmc_summary(fetch_db, write_db, copy_fetch_db, copy_write_db, bench_json, wo
         scale[name] = gib / (1024.0 * float(np.median(v)))
     with open(bench_json) as f:
         bj = json.loads(f.read().strip().splitlines()[-1])
-    algo = {c: bj["kernels"][c]["bytes_per_launch"] for c in ("A", "F1", "F2") if c in bj.get("kernels", {})}
+    kern = bj.get("kernels", {})
+    algo = {c: kern[c]["bytes_per_launch"] for c in ("A", "F1", "F2") if c in kern}
+    fmtb = {c: kern[c].get("format_bytes_per_launch", kern[c]["bytes_per_launch"]) for c in algo}
     out, rows = {}, []
     sums = {}
     for name, db in (("fetch", fetch_db), ("write", write_db)):
@@ -119,7 +122,8 @@ def pmc_summary(fetch_db, write_db, copy_fetch_db, copy_write_db, bench_json, wo
         if "fetch" not in d or "write" not in d:
             continue
         t = d["fetch"][0] + d["write"][0]
-        rows.append((frag.split("<")[0], cat, d["fetch"][1], d["fetch"][0], d["write"][0], t, algo.get(cat)))
+        rows.append((frag.split("<")[0], cat, d["fetch"][1], d["fetch"][0], d["write"][0], t, fmtb.get(cat),
+                     algo.get(cat)))
         if cat in algo:
             out.setdefault(cat, []).append(t)
     tr = {c: float(np.mean(v)) for c, v in out.items()}
@@ -129,12 +133,14 @@ def pmc_summary(fetch_db, write_db, copy_fetch_db, copy_write_db, bench_json, wo
     with open(path, "w") as f:
         json.dump(tr, f, indent=1)
     print(f"scale factors: fetch x{scale['fetch']:.3f}, write x{scale['write']:.3f}")
-    print("| kernel | category | dispatches | fetch MB | write MB | traffic MB | compulsory MB | traffic / compulsory |")
-    print("|---|---|---|---|---|---|---|---|")
-    for k, cat, nd, fe, wr, t, a in rows:
-        ratio = f"{t / a:.3f}" if a else "-"
+    print("| kernel | category | dispatches | fetch MB | write MB | traffic MB | compulsory MB | traffic / compulsory "
+          "| CSR algorithmic MB |")
+    print("|---|---|---|---|---|---|---|---|---|")
+    for k, cat, nd, fe, wr, t, fb, a in rows:
+        ratio = f"{t / fb:.3f}" if fb else "-"
+        fm = f"{fb / 1e6:.1f}" if fb else "-"
         am = f"{a / 1e6:.1f}" if a else "-"
-        print(f"| {k} | {cat} | {nd} | {fe / 1e6:.1f} | {wr / 1e6:.1f} | {t / 1e6:.1f} | {am} | {ratio} |")
+        print(f"| {k} | {cat} | {nd} | {fe / 1e6:.1f} | {wr / 1e6:.1f} | {t / 1e6:.1f} | {fm} | {ratio} | {am} |")
     print(f"wrote {path}")
 
 

@@ -144,6 +144,70 @@ def pmc_summary(fetch_db, write_db, copy_fetch_db, copy_write_db, bench_json, wo
     print(f"wrote {path}")
 
 
+def pmc_block_summary(fetch_db, write_db, copy_fetch_db, copy_write_db, workload):
+    """HBM bytes per dispatch of the 16-column FP32 block kernels from two rocprofv3 --pmc runs of the
+    block bench (FETCH_SIZE, WRITE_SIZE), scaled by a plain 1 GiB device copy as in pmc_summary.  The A
+    product the block roofline times is a pair of dispatches: the long-row pass (spmm_long, BEpiStore)
+    and the diagonal kernel (spmm_dia, BEpiStore) right after it; those pairs come only from the
+    bench's SpMM timing of A.  Writes profiles/traffic_<workload>.json ({"A": bytes per A product})
+    and prints the table."""
+    import re
+    import sqlite3
+
+    def rows_of(db):
+        con = sqlite3.connect(db)
+        rows = con.execute("select dispatch_id, kernel_name, value from counters_collection "
+                           "order by dispatch_id").fetchall()
+        con.close()
+        return rows
+
+    gib = float(2 ** 30)
+    scale = {}
+    for name, db in (("fetch", copy_fetch_db), ("write", copy_write_db)):
+        v = [val for _, k, val in rows_of(db) if "copyBuffer" in k and val * 1024.0 > 0.25 * gib]
+        if not v:
+            raise SystemExit(f"no 1 GiB copy dispatch in {db}")
+        scale[name] = gib / (1024.0 * float(np.median(v)))
+    long_k, dia_k = "spmm_long<float, float, float, 16, cm::BEpiStore<float> >", \
+        "spmm_dia<float, float, float, 16, cm::BEpiStore<float> >"
+    per = {}
+    pairs = {}
+    for name, db in (("fetch", fetch_db), ("write", write_db)):
+        rows = rows_of(db)
+        for _, k, val in rows:
+            if "16" in k and ("spmm_" in k or "k_block_" in k) and "float" in k:
+                per.setdefault(re.sub(r"\(.*$", "", k), {}).setdefault(name, []).append(1024.0 * val * scale[name])
+        a_long, a_dia = [], []
+        for (_, k0, v0), (_, k1, v1) in zip(rows, rows[1:]):
+            if long_k in k0 and dia_k in k1:
+                a_long.append(1024.0 * v0 * scale[name])
+                a_dia.append(1024.0 * v1 * scale[name])
+        pairs[name] = (a_long, a_dia)
+    if not pairs["fetch"][0] or not pairs["write"][0]:
+        raise SystemExit("no A product (spmm_long then spmm_dia, BEpiStore) in the databases")
+    lf, df = (float(np.mean(v)) for v in pairs["fetch"])
+    lw, dw = (float(np.mean(v)) for v in pairs["write"])
+    tr = {"A": lf + df + lw + dw,
+          "_note": f"HBM bytes per 16-column FP32 A product (spmm_long + spmm_dia) from rocprofv3 --pmc FETCH_SIZE "
+                   f"(x{scale['fetch']:.3f}) + WRITE_SIZE (x{scale['write']:.3f}), scaled by a 1 GiB device copy; "
+                   f"{len(pairs['fetch'][0])} products"}
+    path = os.path.join(ROOT, "profiles", f"traffic_{workload}.json")
+    with open(path, "w") as f:
+        json.dump(tr, f, indent=1)
+    print(f"scale factors: fetch x{scale['fetch']:.3f}, write x{scale['write']:.3f}")
+    print("| kernel | dispatches | fetch MB | write MB | traffic MB |")
+    print("|---|---|---|---|---|")
+    for k, d in sorted(per.items(), key=lambda kv: -np.mean(kv[1].get("fetch", [0]))):
+        if "fetch" not in d or "write" not in d:
+            continue
+        fe, wr = float(np.mean(d["fetch"])), float(np.mean(d["write"]))
+        print(f"| `{k}` | {len(d['fetch'])} | {fe / 1e6:.1f} | {wr / 1e6:.1f} | {(fe + wr) / 1e6:.1f} |")
+    print(f"A product ({len(pairs['fetch'][0])} timed): long-row pass {(lf + lw) / 1e6:.1f} MB "
+          f"(fetch {lf / 1e6:.1f}), diagonal kernel {(df + dw) / 1e6:.1f} MB (fetch {df / 1e6:.1f}), "
+          f"total {tr['A'] / 1e6:.1f} MB")
+    print(f"wrote {path}")
+
+
 def trace_summary(db, thr_us=5.0, top=25):
     """Per-kernel durations from a rocprofv3 --kernel-trace result database, as a markdown table:
     launches, share of GPU time, mean duration, and the launches longer than thr_us (no-op launches
@@ -377,6 +441,10 @@ def bench_block(args, w):
                               "format_GBs": round(by_h / us_h / 1e3, 1)}},
         "single_rhs_spmv_formats": {str(b_): f for b_, f in formats.items()} if tuned else None,
     }
+    traffic_file = os.path.join(ROOT, "profiles", f"traffic_{args.workload}.json")
+    if os.path.exists(traffic_file):
+        with open(traffic_file) as f:
+            out["roofline"]["traffic"] = json.load(f).get("A")
     if not args.no_cpu:
         out["cpu_baseline"], x_or = cpu_baseline(w, A, B[:, 0], F, args.cpu_seconds)
         out["rel_error_vs_oracle_column0"] = {"mixed_block": _rel(Xmx[:, 0], x_or), "fp64_single": _rel(X64[:, 0], x_or)}
@@ -642,7 +710,10 @@ def main():
         trace_summary(args.trace_summary)
         return
     if args.pmc_summary:
-        pmc_summary(*args.pmc_summary, args.workload)
+        if args.workload == "st27-100-rhs16":
+            pmc_block_summary(*args.pmc_summary[:4], args.workload)
+        else:
+            pmc_summary(*args.pmc_summary, args.workload)
         return
     if args.calibrate_copy:
         import torch

@@ -147,10 +147,11 @@ def pmc_summary(fetch_db, write_db, copy_fetch_db, copy_write_db, bench_json, wo
 def pmc_block_summary(fetch_db, write_db, copy_fetch_db, copy_write_db, workload):
     """HBM bytes per dispatch of the 16-column FP32 block kernels from two rocprofv3 --pmc runs of the
     block bench (FETCH_SIZE, WRITE_SIZE), scaled by a plain 1 GiB device copy as in pmc_summary.  The A
-    product the block roofline times is a pair of dispatches: the long-row pass (spmm_long, BEpiStore)
-    and the diagonal kernel (spmm_dia, BEpiStore) right after it; those pairs come only from the
-    bench's SpMM timing of A.  Writes profiles/traffic_<workload>.json ({"A": bytes per A product})
-    and prints the table."""
+    product the block roofline times is the diagonal kernel (spmm_dia, BEpiStore) with the long-row
+    dispatches right before it: the segment sums (spmm_short, BEpiStore) and their tile reduction
+    (k_long_tiles), or one wave-per-row pass (spmm_long) when the remainders are not tiled.  Only the
+    bench's SpMM timing of A launches that sequence.  Writes profiles/traffic_<workload>.json ({"A":
+    bytes per A product}) and prints the table."""
     import re
     import sqlite3
 
@@ -168,27 +169,35 @@ def pmc_block_summary(fetch_db, write_db, copy_fetch_db, copy_write_db, workload
         if not v:
             raise SystemExit(f"no 1 GiB copy dispatch in {db}")
         scale[name] = gib / (1024.0 * float(np.median(v)))
-    long_k, dia_k = "spmm_long<float, float, float, 16, cm::BEpiStore<float> >", \
-        "spmm_dia<float, float, float, 16, cm::BEpiStore<float> >"
+    dia_k = "spmm_dia<float, float, float, 16, cm::BEpiStore<float> >"
+    long_k = "spmm_long<float, float, float, 16, cm::BEpiStore<float> >"
+    seg_k, tiles_k = "spmm_short<float, float, float, 16, cm::BEpiStore<float> >", "k_long_tiles<float, float>"
     per = {}
     pairs = {}
     for name, db in (("fetch", fetch_db), ("write", write_db)):
         rows = rows_of(db)
         for _, k, val in rows:
-            if "16" in k and ("spmm_" in k or "k_block_" in k) and "float" in k:
+            if ("16" in k and ("spmm_" in k or "k_block_" in k) and "float" in k) or tiles_k in k:
                 per.setdefault(re.sub(r"\(.*$", "", k), {}).setdefault(name, []).append(1024.0 * val * scale[name])
         a_long, a_dia = [], []
-        for (_, k0, v0), (_, k1, v1) in zip(rows, rows[1:]):
-            if long_k in k0 and dia_k in k1:
-                a_long.append(1024.0 * v0 * scale[name])
-                a_dia.append(1024.0 * v1 * scale[name])
+        for i in range(2, len(rows)):
+            if dia_k not in rows[i][1]:
+                continue
+            if tiles_k in rows[i - 1][1] and seg_k in rows[i - 2][1]:
+                lv = rows[i - 1][2] + rows[i - 2][2]
+            elif long_k in rows[i - 1][1]:
+                lv = rows[i - 1][2]
+            else:
+                continue
+            a_long.append(1024.0 * lv * scale[name])
+            a_dia.append(1024.0 * rows[i][2] * scale[name])
         pairs[name] = (a_long, a_dia)
     if not pairs["fetch"][0] or not pairs["write"][0]:
-        raise SystemExit("no A product (spmm_long then spmm_dia, BEpiStore) in the databases")
+        raise SystemExit("no A product (long-row dispatches, then spmm_dia, BEpiStore) in the databases")
     lf, df = (float(np.mean(v)) for v in pairs["fetch"])
     lw, dw = (float(np.mean(v)) for v in pairs["write"])
     tr = {"A": lf + df + lw + dw,
-          "_note": f"HBM bytes per 16-column FP32 A product (spmm_long + spmm_dia) from rocprofv3 --pmc FETCH_SIZE "
+          "_note": f"HBM bytes per 16-column FP32 A product (long-row dispatches + spmm_dia) from rocprofv3 --pmc FETCH_SIZE "
                    f"(x{scale['fetch']:.3f}) + WRITE_SIZE (x{scale['write']:.3f}), scaled by a 1 GiB device copy; "
                    f"{len(pairs['fetch'][0])} products"}
     path = os.path.join(ROOT, "profiles", f"traffic_{workload}.json")

@@ -416,6 +416,21 @@ __global__ __launch_bounds__(BS) void spmm_long(int cnt, const int *__restrict__
     block_colgroup_partials<Epi::K, KB, CPL>(acc, red, partials, slot0 + blockIdx.x);
 }
 
+// Long remainders cut into column tiles: ne = (long rows) x KB sums, each
+// the total of its ntile segment sums (segment t at seg[t * ne + e]), added
+// in tile order.
+template <class T, class Acc>
+__global__ __launch_bounds__(BS) void k_long_tiles(long long ne, int ntile, const T *__restrict__ seg,
+                                                   T *__restrict__ racc, const BlockFlags *fl)
+{
+    if (fl && fl->all_done) return;
+    for (long long e = (long long)blockIdx.x * BS + threadIdx.x; e < ne; e += (long long)gridDim.x * BS) {
+        Acc s = (Acc)seg[e];
+        for (int t = 1; t < ntile; t++) s += (Acc)seg[(size_t)t * ne + e];
+        racc[e] = (T)s;
+    }
+}
+
 // First stage of a two-stage column reduction: workgroup g sums partial
 // slots [g * per, (g + 1) * per) of every column (same thread / column
 // layout and fixed order as k_block_finalize) into slot g of out.

@@ -225,16 +225,21 @@ struct DiaIndex {
     long long nrem = 0;
     int *off = nullptr, *dsrc = nullptr, *rptr = nullptr, *rcol = nullptr, *rsrc = nullptr;
     // Rows whose remainder is longer than SHORT_MAX, for the block products:
-    // lpos[row] = their index in a compact CSR of those remainders (lptr /
-    // lcol, values gathered through lsrc), -1 for the other rows; lrows =
-    // 0..nlong-1 (the row list of the wave-per-row kernel).
-    int nlong = 0;
-    long long nlrem = 0;
+    // lpos[row] = their index q (0..nlong-1), -1 for the other rows.  Their
+    // remainders are cut into ntile column tiles of tw columns and stored as a
+    // CSR of segments, tile-major: segment t * nlong + q holds row q's entries
+    // with columns in [t * tw, (t + 1) * tw) (lptr / lcol, values gathered
+    // through lsrc).  lrows = 0..nseg-1 (the segment list of the short-row
+    // kernel that sums them).  Walking the segments in order keeps the x rows
+    // that an XCD's waves in flight gather within about one tile, which its
+    // L2 holds.
+    int nlong = 0, ntile = 1, tw = 0;
+    long long nlrem = 0, nseg = 0;
     int *lpos = nullptr, *lptr = nullptr, *lcol = nullptr, *lsrc = nullptr, *lrows = nullptr;
     size_t bytes() const
     {
         return ((size_t)nd + (size_t)nd * ldn + (rptr ? (size_t)n + 1 : 0) + 2 * (size_t)nrem +
-                (nlong ? (size_t)n + 2 * (size_t)nlong + 1 + 2 * (size_t)nlrem : 0)) *
+                (nlong ? (size_t)n + 2 * (size_t)nseg + 1 + 2 * (size_t)nlrem : 0)) *
                sizeof(int);
     }
     ~DiaIndex()
@@ -257,6 +262,10 @@ struct DiaIndex {
 // diagonals with a CSR remainder.
 enum SpmvFormat { FMT_STAGED = 0, FMT_STAGED16 = 1, FMT_SLOT = 2, FMT_SLOT16 = 3, FMT_DIA = 4, FMT_DIA2 = 5 };
 constexpr double DIA_FILL = 0.6;  // a diagonal is stored densely when at least this share of rows has it
+// Column tiles of the long remainders of the diagonal block products: about
+// one tile per LONG_TILE_COLS columns (2 MB of x rows at 16 FP32 columns),
+// at most LONG_TILES_MAX.
+constexpr int LONG_TILE_COLS = 32768, LONG_TILES_MAX = 32;
 
 // Rows per thread of spmv_dia: FP16 values are read two rows at a time (one
 // 4-byte load per diagonal); FP32 and FP64 one row per thread.  The second
@@ -572,28 +581,47 @@ static int build_dia(DevCsr<double> &m, hipStream_t s)
         }
         rptr[i + 1] = (int)rcol.size();
     }
-    // long remainders (block products): a compact CSR of the rows with more
-    // than SHORT_MAX remainder entries
-    std::vector<int> lpos(n, -1), lptr{0}, lcol, lsrc;
-    for (int i = 0; i < n; i++) {
-        if (rptr[i + 1] - rptr[i] <= SHORT_MAX) continue;
-        lpos[i] = (int)lptr.size() - 1;
-        for (int k = rptr[i]; k < rptr[i + 1]; k++) {
-            lcol.push_back(rcol[k]);
-            lsrc.push_back(rsrc[k]);
+    // long remainders (block products): the rows with more than SHORT_MAX
+    // remainder entries, as a tile-major CSR of (column tile, row) segments
+    // (counting sort by tile; entries keep their row order in a segment)
+    std::vector<int> lpos(n, -1), lrow;
+    for (int i = 0; i < n; i++)
+        if (rptr[i + 1] - rptr[i] > SHORT_MAX) {
+            lpos[i] = (int)lrow.size();
+            lrow.push_back(i);
         }
-        lptr.push_back((int)lcol.size());
+    const int nlong = (int)lrow.size();
+    const int ntile = std::min(LONG_TILES_MAX, std::max(1, (n + LONG_TILE_COLS - 1) / LONG_TILE_COLS));
+    const int tw = (n + ntile - 1) / ntile;
+    const long long nseg = (long long)ntile * nlong;
+    std::vector<int> lptr((size_t)nseg + 1, 0), lcol, lsrc;
+    for (int q = 0; q < nlong; q++)
+        for (int k = rptr[lrow[q]]; k < rptr[lrow[q] + 1]; k++) lptr[(size_t)(rcol[k] / tw) * nlong + q + 1]++;
+    for (long long g = 0; g < nseg; g++) lptr[g + 1] += lptr[g];
+    lcol.resize(lptr[nseg]);
+    lsrc.resize(lptr[nseg]);
+    {
+        std::vector<int> fill(lptr.begin(), lptr.end() - 1);
+        for (int q = 0; q < nlong; q++)
+            for (int k = rptr[lrow[q]]; k < rptr[lrow[q] + 1]; k++) {
+                const int pos = fill[(size_t)(rcol[k] / tw) * nlong + q]++;
+                lcol[pos] = rcol[k];
+                lsrc[pos] = rsrc[k];
+            }
     }
     auto D = std::make_shared<DiaIndex>();
     D->n = n;
     D->ldn = ldn;
     D->nd = nd;
     D->nrem = (long long)rcol.size();
-    D->nlong = (int)lptr.size() - 1;
+    D->nlong = nlong;
+    D->ntile = ntile;
+    D->tw = tw;
+    D->nseg = nseg;
     D->nlrem = (long long)lcol.size();
     if (D->nlong) {
-        std::vector<int> iota(D->nlong);
-        for (int q = 0; q < D->nlong; q++) iota[q] = q;
+        std::vector<int> iota((size_t)nseg);
+        for (long long g = 0; g < nseg; g++) iota[g] = (int)g;
         CM_TRY(dev_upload(&D->lpos, lpos.data(), lpos.size(), s));
         CM_TRY(dev_upload(&D->lptr, lptr.data(), lptr.size(), s));
         CM_TRY(dev_upload(&D->lcol, lcol.data(), lcol.size(), s));
@@ -1617,8 +1645,9 @@ struct BlockKrylov {
     int ensure_racc()
     {
         size_t need = 0;
-        auto one = [&](const auto *m) {
-            if (m && m->block_dia && m->dia) need = std::max(need, (size_t)m->dia->nlong * KB);
+        auto one = [&](const auto *m) {  // the row sums, then the segment sums when tiled
+            if (m && m->block_dia && m->dia)
+                need = std::max(need, (size_t)m->dia->nlong * KB * (m->dia->ntile > 1 ? 1 + m->dia->ntile : 1));
         };
         one(A);
         one(G);
@@ -1686,10 +1715,17 @@ struct BlockKrylov {
         if (m.block_dia && m.dval) {
             const DiaIndex &D = *m.dia;
             const int gd = dia_grid(n);
-            if (D.nlong) {  // long remainders first, a wave per row, sums into racc
-                const int gl = std::min((D.nlong + (BS / WAVE) - 1) / (BS / WAVE), SPMM_CAP);
-                spmm_long<V, T, Acc, KB, BEpiStore<T>><<<gl, BS, 0, stream>>>(
-                    D.nlong, D.lrows, D.lptr, D.lcol, m.lval, nullptr, xin, BEpiStore<T>{racc}, gf, partials, 0);
+            if (D.nlong) {  // long remainders first: segment sums, added by tile into racc
+                const long long nseg = D.nseg;
+                const int gseg = (int)std::min<long long>((nseg + SROWS - 1) / SROWS, SPMM_CAP);
+                T *seg = D.ntile > 1 ? racc + (size_t)D.nlong * KB : racc;
+                spmm_short<V, T, Acc, KB, BEpiStore<T>><<<gseg, BS, 0, stream>>>(
+                    (int)nseg, D.lrows, D.lptr, D.lcol, m.lval, nullptr, xin, BEpiStore<T>{seg}, gf, partials, 0);
+                if (D.ntile > 1) {
+                    const long long ne_l = (long long)D.nlong * KB;
+                    const int gt = (int)std::min<long long>((ne_l + BS - 1) / BS, 8192);
+                    k_long_tiles<T, Acc><<<gt, BS, 0, stream>>>(ne_l, D.ntile, seg, racc, gf);
+                }
             }
             spmm_dia<V, T, Acc, KB, Epi><<<gd, BS, 0, stream>>>(n, D.ldn, D.nd, D.off, m.dval, D.rptr, D.rcol, m.rval,
                                                                D.nlong ? D.lpos : nullptr, racc, m.rowscale, xin,
@@ -1730,8 +1766,11 @@ struct BlockKrylov {
         *avg_us = 1e3 * ms / reps;
         if (m.block_dia && m.dval) {  // diagonals (zeros included) and remainder once, X and Y once
             const DiaIndex &D = *m.dia;
+            // (+ lpos, the segment pointers and list, and the segment sums
+            // written and read back when the long remainders are tiled)
             *bytes = (double)D.nd * m.n * sizeof(V) + 4.0 * D.nd + (double)D.nrem * (sizeof(V) + sizeof(int)) +
-                     (D.rptr ? 4.0 * (m.n + 1) : 0.0) + (D.nlong ? 4.0 * m.n + 8.0 * D.nlong : 0.0) +
+                     (D.rptr ? 4.0 * (m.n + 1) : 0.0) + (D.nlong ? 4.0 * m.n + 8.0 * D.nseg : 0.0) +
+                     (D.ntile > 1 ? 2.0 * D.nseg * KB * sizeof(T) : 0.0) +
                      2.0 * (double)ne() * sizeof(T) + (m.rowscale ? 4.0 * m.n : 0.0);
         } else {
             *bytes = (double)m.nnz * (sizeof(V) + sizeof(int)) + 4.0 * (m.n + 1) + 4.0 * (bn.ns + bn.nl) +

@@ -435,8 +435,9 @@ def bench_block(args, w):
         "fp64_block": {"value": round(v64b, 3), "unit": "solves/s", "iterations": [int(i) for i in r64b[1]]},
         "speedup_vs_fp64_one_at_a_time": round(value / v64, 3),
         "roofline": {"bound": "hbm", "kernel": "A block SpMM over 16 FP32 columns: spmm_dia<float, float, float, 16> "
-                                               "plus spmm_long for the rows with long remainders (row-binned "
-                                               "spmm_short/spmm_long when the diagonal block format does not apply)",
+                                               "after the long rows' remainders as column-tiled segments "
+                                               "(spmm_short, then k_long_tiles); row-binned spmm_short/spmm_long "
+                                               "when the diagonal block format does not apply",
                      "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": None, "bytes_per_launch": alg,
                      "avg_us": round(us, 2), "timing": "20 back-to-back SpMMs between HIP events, separate "

@@ -271,8 +271,8 @@ __global__ __launch_bounds__(BS) void spmm_short(int cnt, const int *__restrict_
 // segments and no column index is read for the diagonal part.  Row sums in
 // diagonal order, then the remainder.  Rows with long remainders (lpos[row]
 // >= 0; lpos null when there are none) take their remainder sums from racc
-// (KB values per long row, written by a wave-per-row pass before this one)
-// instead of walking the remainder themselves.
+// (KB values per long row, written before this launch from the remainders'
+// column-tiled segments) instead of walking the remainder themselves.
 template <class V, class T, class Acc, int KB, class Epi>
 __global__ __launch_bounds__(BS) void spmm_dia(int n, int ldn, int nd, const int *__restrict__ off,
                                                const V *__restrict__ dval, const int *__restrict__ rptr,

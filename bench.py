@@ -485,6 +485,10 @@ def _serve_systems(nsys, seed):
         return list(ex.map(one, specs))
 
 
+def _progress(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
 def bench_serve(args, w):
     """Serving workload: Poisson arrivals of solve requests over resident systems.  The
     served path (continuous batching, FP32 inner / FP64 refinement) and the
@@ -554,6 +558,7 @@ def bench_serve(args, w):
     # served: continuous batching, mixed precision
     srv = cmamd.Server(slots=args.serve_slots, precision=32, small_n=args.serve_small_n)
     sid = [srv.add_system(h) for h in handles]
+    _progress(f"{len(handles)} systems uploaded and registered")
     t0 = time.perf_counter()
     for kw in (1, 2, 4, 8, 16):  # create every system's block workspaces once (not timed)
         ids = [srv.submit(sid[s], np.tile(pools[s][1], (1, 2))[:, :kw], tol=w["tol"], nmax=w["nmax"])
@@ -562,11 +567,15 @@ def bench_serve(args, w):
             srv.wait(r)
     t_warm = time.perf_counter() - t0
     st0 = srv.stats()
+    _progress(f"block workspaces of {len(systems)} systems created in {t_warm:.1f}s; replaying {len(reqs)} requests")
     served_raw, t_served = replay(lambda s, B: srv.submit(sid[s], B, tol=w["tol"], nmax=w["nmax"]), srv.wait)
     st = srv.stats()
+    _progress(f"served in {t_served:.1f}s; checking every column against b on the host")
     served = []
     info_counts, failed = {}, []
     for i, r in enumerate(served_raw):
+        if i and i % 2000 == 0:
+            _progress(f"checked {i} requests")
         wr, wx = check(i, r.X)
         served.append((r.latency_ms, r.queue_ms, r.batch_cols, int(np.count_nonzero(r.info)), wr, wx,
                        r.X if i % SAMPLE == 0 else None, r.service_ms))
@@ -578,6 +587,7 @@ def bench_serve(args, w):
                            "info": [int(c) for c in r.info], "resval": [float(v) for v in r.resval],
                            "outer": r.outer, "inner": r.inner, "true_rel_residual": wr})
     del served_raw
+    _progress("checked; autotuning the single-RHS formats for the FP64 baseline")
     for key in ("batches", "columns", "persistent_batches", "persistent_columns", "evictions"):
         st[key] -= st0[key]
     srv.close()
@@ -588,6 +598,7 @@ def bench_serve(args, w):
         for h in handles:
             h.autotune(reps=3)
     t_tune = time.perf_counter() - t_tune
+    _progress(f"autotuned in {t_tune:.1f}s; FP64 one-at-a-time baseline")
     lock = threading.Lock()
     queue = []
     cv = threading.Condition(lock)
@@ -613,6 +624,8 @@ def bench_serve(args, w):
                 info.append(inf)
             te = time.perf_counter()
             out[i] = (X if i % SAMPLE == 0 else None, info, 1e3 * (ts - t_arr), 1e3 * (te - t_arr))
+            if i and i % 2000 == 0:
+                _progress(f"FP64 baseline: {i} requests solved")
     nb = len(reqs) if args.serve_fp64_requests <= 0 else min(len(reqs), args.serve_fp64_requests)
     base = [None] * nb
     th = threading.Thread(target=fp64_worker, args=(base,))

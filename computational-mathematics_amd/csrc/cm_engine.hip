@@ -27,6 +27,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <climits>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -591,7 +592,8 @@ static int build_dia(DevCsr<double> &m, hipStream_t s)
             lrow.push_back(i);
         }
     const int nlong = (int)lrow.size();
-    const int ntile = std::min(LONG_TILES_MAX, std::max(1, (n + LONG_TILE_COLS - 1) / LONG_TILE_COLS));
+    int ntile = std::min(LONG_TILES_MAX, std::max(1, (n + LONG_TILE_COLS - 1) / LONG_TILE_COLS));
+    if ((long long)ntile * nlong > INT_MAX) ntile = 1;  // segment ids are int
     const int tw = (n + ntile - 1) / ntile;
     const long long nseg = (long long)ntile * nlong;
     std::vector<int> lptr((size_t)nseg + 1, 0), lcol, lsrc;

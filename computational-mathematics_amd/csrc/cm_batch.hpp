@@ -131,10 +131,10 @@ __device__ void wg_bicgstab(const BatchSys &S, BatchItem &it, double *w, double 
         cycles = cyc + 1;
         if (cyc == 0) {
             ro0 = ro;
-            if (it.nmax <= 0) {
+            if (it.nmax <= 0) {  // NMAX < 0: the reference leaves info as the caller set it (-1 here)
                 resval = ro / ro0;
                 iter = 0;
-                info = it.nmax == 0 ? 3 : 0;
+                info = it.nmax == 0 ? 3 : -1;
                 break;
             }
         } else if (ro < it.tol * ro0) {  // true residual below tol: done

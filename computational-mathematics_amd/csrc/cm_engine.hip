@@ -266,7 +266,7 @@ constexpr double DIA_FILL = 0.6;  // a diagonal is stored densely when at least 
 // Column tiles of the long remainders of the diagonal block products: about
 // one tile per LONG_TILE_COLS columns (2 MB of x rows at 16 FP32 columns),
 // at most LONG_TILES_MAX.
-constexpr int LONG_TILE_COLS = 32768, LONG_TILES_MAX = 32;
+constexpr int LONG_TILE_COLS = 32768, LONG_TILES_MAX = 32, LONG_SEG_MIN = 8;
 
 // Rows per thread of spmv_dia: FP16 values are read two rows at a time (one
 // 4-byte load per diagonal); FP32 and FP64 one row per thread.  The second
@@ -306,6 +306,7 @@ struct DevCsr {
     bool dia2 = false;                 // ... in its second variant (FMT_DIA2)
     bool block_dia = false;            // block products (SpMM) through spmm_dia
     int nlev = 0;
+    bool lower = false;  // the level schedule's triangle (lower: rows read smaller columns)
     int *lrows = nullptr;
     std::vector<int> loff;
     bool present() const { return ptr != nullptr; }
@@ -401,6 +402,7 @@ static int cast_csr(DevCsr<V> &dst, const DevCsr<double> &src, hipStream_t s)
     dst.col = src.col;
     dst.blk = src.blk;
     dst.nlev = src.nlev;
+    dst.lower = src.lower;
     dst.lrows = src.lrows;
     dst.loff = src.loff;
     CM_TRY(dev_alloc(&dst.val, (size_t)dst.nnz + CSR_PAD));
@@ -592,7 +594,13 @@ static int build_dia(DevCsr<double> &m, hipStream_t s)
             lrow.push_back(i);
         }
     const int nlong = (int)lrow.size();
+    long long nlong_rem = 0;
+    for (int q = 0; q < nlong; q++) nlong_rem += rptr[lrow[q] + 1] - rptr[lrow[q]];
     int ntile = std::min(LONG_TILES_MAX, std::max(1, (n + LONG_TILE_COLS - 1) / LONG_TILE_COLS));
+    // at least LONG_SEG_MIN entries per segment on average: bounds the
+    // segment sums (written and read back by every block product, and held
+    // in the block workspace) by the long remainders' own size
+    if (nlong) ntile = (int)std::min<long long>(ntile, std::max<long long>(1, nlong_rem / ((long long)LONG_SEG_MIN * nlong)));
     if ((long long)ntile * nlong > INT_MAX) ntile = 1;  // segment ids are int
     const int tw = (n + ntile - 1) / ntile;
     const long long nseg = (long long)ntile * nlong;
@@ -680,6 +688,7 @@ static int build_levels(DevCsr<double> &m, const int *ptr, const int *col, bool 
         nlev = std::max(nlev, l + 1);
     }
     m.nlev = nlev;
+    m.lower = lower;
     m.loff.assign(nlev + 1, 0);
     for (int i = 0; i < n; i++) m.loff[lev[i] + 1]++;
     for (int l = 0; l < nlev; l++) m.loff[l + 1] += m.loff[l];
@@ -1090,7 +1099,7 @@ struct Krylov {
             const int cnt = m.loff[l + 1] - m.loff[l];
             const int g = (cnt + BS - 1) / BS;
             timed(KC_LU, (double)cnt * 3.0 * sizeof(T), [&] {
-                k_trsv_level<T, VF><<<g, BS, 0, stream>>>(cnt, m.lrows + m.loff[l], m.ptr, m.col, m.val,
+                k_trsv_level<T, VF><<<g, BS, 0, stream>>>(cnt, m.lower, m.lrows + m.loff[l], m.ptr, m.col, m.val,
                                                           m.rowscale, bin, xout, use_state ? st : nullptr);
             });
         }
@@ -1593,6 +1602,7 @@ struct BlockKrylov {
     size_t partial_slots = 0;
     T *racc = nullptr;  // long-row remainder sums of the diagonal block products (KB per long row)
     size_t racc_cap = 0;
+    bool racc_short = false;  // a product was refused: racc smaller than it needs (nothing launched)
     hipEvent_t ev[2] = {nullptr, nullptr};
     bool use_graph = true;
     hipGraphExec_t g1 = nullptr, gN = nullptr;  // one pass, GRAPH_PASSES passes
@@ -1642,18 +1652,24 @@ struct BlockKrylov {
     static int long_grid(const RowBins &bn) { return std::min((bn.nl + (BS / WAVE) - 1) / (BS / WAVE), SPMM_CAP); }
     static int dia_grid(int nrows) { return std::min((nrows + SROWS - 1) / SROWS, SPMM_CAP); }
 
+    // Elements of the long-row remainder sums one product of m needs: the
+    // row sums, then the segment sums when its remainders are tiled.
+    template <class V>
+    static size_t racc_of(const DevCsr<V> *m)
+    {
+        if (!m || !m->block_dia || !m->dia || !m->dia->nlong) return 0;
+        const DiaIndex &D = *m->dia;
+        return (size_t)D.nlong * KB * (D.ntile > 1 ? 1 + (size_t)D.ntile : 1);
+    }
+    size_t racc_need() const { return std::max(racc_of(A), std::max(racc_of(G), racc_of(Hm))); }
+
     // Room for the long-row remainder sums of the matrices that use the
-    // diagonal block products (called outside graph capture, before a solve).
+    // diagonal block products.  Called outside graph capture before every
+    // solve, timing and tuning pass (a matrix may have switched to the
+    // diagonal block products since the workspace was made).
     int ensure_racc()
     {
-        size_t need = 0;
-        auto one = [&](const auto *m) {  // the row sums, then the segment sums when tiled
-            if (m && m->block_dia && m->dia)
-                need = std::max(need, (size_t)m->dia->nlong * KB * (m->dia->ntile > 1 ? 1 + m->dia->ntile : 1));
-        };
-        one(A);
-        one(G);
-        one(Hm);
+        const size_t need = racc_need();
         if (need <= racc_cap) return CM_OK;
         CM_HIP(hipStreamSynchronize(stream));
         drop_graphs();  // their launches carry the old buffer
@@ -1665,13 +1681,16 @@ struct BlockKrylov {
     }
 
     // device bytes of a block solver of this type for an n-row system whose
-    // three matrices have (short + long) row-bin workgroups `slots`
-    static size_t bytes_for(int n, size_t slots)
+    // three matrices have (short + long) row-bin workgroups `slots` and whose
+    // long-row remainder sums take `racc` elements
+    static size_t bytes_for(int n, size_t slots, size_t racc)
     {
         const size_t e = (size_t)n * KB;
         return 11 * e * sizeof(T) + KB * sizeof(SolveState) + sizeof(BlockFlags) +
-               std::max(slots, (size_t)MAX_RED_BLOCKS) * 4 * KB * sizeof(double) + 3 * (size_t)n * sizeof(int);
+               std::max(slots, (size_t)MAX_RED_BLOCKS) * 4 * KB * sizeof(double) + 3 * (size_t)n * sizeof(int) +
+               racc * sizeof(T);
     }
+    size_t bytes() const { return bytes_for(n, partial_slots, racc_cap); }
 
     template <class KR>
     int create(const KR &k, hipStream_t strm)
@@ -1717,6 +1736,10 @@ struct BlockKrylov {
         if (m.block_dia && m.dval) {
             const DiaIndex &D = *m.dia;
             const int gd = dia_grid(n);
+            if (racc_of(&m) > racc_cap) {  // never launch through a missing or short buffer
+                racc_short = true;
+                return;
+            }
             if (D.nlong) {  // long remainders first: segment sums, added by tile into racc
                 const long long nseg = D.nseg;
                 const int gseg = (int)std::min<long long>((nseg + SROWS - 1) / SROWS, SPMM_CAP);
@@ -1750,9 +1773,19 @@ struct BlockKrylov {
     // branch) over the KB columns, bracketed by events on this stream; bytes =
     // compulsory traffic of one product (matrix and row lists once, x and y
     // once per column).
+    // error when a product was refused for want of racc room (clears the flag)
+    int check_racc()
+    {
+        if (!racc_short) return CM_OK;
+        racc_short = false;
+        drop_graphs();
+        return fail(CM_ERR_HIP, "block product refused: long-row buffer smaller than the matrix needs");
+    }
+
     template <class V>
     int time_one(const DevCsr<V> &m, const RowBins &bn, int reps, double *avg_us, double *bytes)
     {
+        CM_TRY(ensure_racc());
         hipEvent_t e0, e1;
         CM_HIP(hipEventCreate(&e0));
         CM_HIP(hipEventCreate(&e1));
@@ -1765,6 +1798,7 @@ struct BlockKrylov {
         CM_HIP(hipEventElapsedTime(&ms, e0, e1));
         (void)hipEventDestroy(e0);
         (void)hipEventDestroy(e1);
+        CM_TRY(check_racc());
         *avg_us = 1e3 * ms / reps;
         if (m.block_dia && m.dval) {  // diagonals (zeros included) and remainder once, X and Y once
             const DiaIndex &D = *m.dia;
@@ -1883,12 +1917,14 @@ struct BlockKrylov {
     {
         if (!use_graph) {
             for (int i = 0; i < size; i++) enqueue_iteration();
+            CM_TRY(check_racc());
             CM_HIP(hipGetLastError());
             return CM_OK;
         }
         if (!g1) {
             CM_TRY(capture(g1, 1));
             CM_TRY(capture(gN, GRAPH_PASSES));
+            CM_TRY(check_racc());
         }
         for (; size >= GRAPH_PASSES; size -= GRAPH_PASSES) CM_HIP(hipGraphLaunch(gN, stream));
         for (; size > 0; size--) CM_HIP(hipGraphLaunch(g1, stream));
@@ -1898,6 +1934,7 @@ struct BlockKrylov {
     // initial per-column states; columns not in `active` start done
     int begin(double tol, int NMAX, const int *active, const double *col_tol = nullptr)
     {
+        CM_TRY(ensure_racc());
         for (int j = 0; j < KB; j++) {
             SolveState &q = hst[j];
             std::memset(&q, 0, sizeof q);
@@ -1929,6 +1966,7 @@ struct BlockKrylov {
     // one sized from the residual contraction of the slowest live column).
     int run(int NMAX, SolveState *fin)
     {
+        CM_TRY(check_racc());  // the setup product
         int qs[2], nq = 0, slot = 0, chunk = 4;
         long long enq = 0;
         const long long nmax_passes = std::max(NMAX, 0);
@@ -1986,6 +2024,7 @@ struct BlockBase {
     virtual void set_stream(hipStream_t s) = 0;
     virtual void set_graphs(bool on) = 0;
     virtual void drop_graphs() = 0;
+    virtual int ensure_workspace() = 0;
     virtual size_t bytes() const = 0;
     virtual int time_spmm(char which, int bits, int reps, double *avg_us, double *bytes) = 0;
     virtual int tune(int bits, int reps, int *chosen) = 0;
@@ -2026,9 +2065,9 @@ struct BlockSolver : BlockBase {
 
     size_t bytes() const override
     {
-        size_t b = decltype(b64)::bytes_for(n, b64.partial_slots) + (size_t)n * KB * sizeof(double);
-        if (b32) b += std::remove_reference<decltype(*b32)>::type::bytes_for(n, b32->partial_slots);
-        if (b16) b += std::remove_reference<decltype(*b16)>::type::bytes_for(n, b16->partial_slots);
+        size_t b = b64.bytes() + (size_t)n * KB * sizeof(double);
+        if (b32) b += b32->bytes();
+        if (b16) b += b16->bytes();
         return b;
     }
 
@@ -2054,6 +2093,15 @@ struct BlockSolver : BlockBase {
         b64.drop_graphs();
         if (b32) b32->drop_graphs();
         if (b16) b16->drop_graphs();
+    }
+
+    // long-row buffers sized for the matrices' current block product formats
+    int ensure_workspace() override
+    {
+        CM_TRY(b64.ensure_racc());
+        if (b32) CM_TRY(b32->ensure_racc());
+        if (b16) CM_TRY(b16->ensure_racc());
+        return CM_OK;
     }
 
     template <class KI, class BI>
@@ -2109,7 +2157,7 @@ struct BlockSolver : BlockBase {
         for (int j = 0; j < ncols; j++) {
             resval[j] = fin[j].rc / fin[j].ro;
             iter[j] = fin[j].iter;
-            info[j] = fin[j].info;
+            if (NMAX >= 0) info[j] = fin[j].info;  // the reference leaves info untouched when NMAX < 0
         }
         return CM_OK;
     }
@@ -2123,6 +2171,7 @@ struct BlockSolver : BlockBase {
                    int *inner_total, int *info)
     {
         bi.ncols = ncols;
+        CM_TRY(b64.ensure_racc());  // the FP64 residual products below
         std::vector<double> res0(ncols, -1.0), prev(ncols, -1.0);
         std::vector<int> act(KB, 0), fin_info(ncols, -1), loose(ncols, 0);
         int steps = 0, inner = 0;
@@ -2137,6 +2186,7 @@ struct BlockSolver : BlockBase {
             b64.hfl[0].ncols = ncols;
             CM_HIP(hipMemcpyAsync(b64.fl, b64.hfl, sizeof(BlockFlags), hipMemcpyHostToDevice, stream));
             b64.spmm(*b64.A, b64.bA, b64.x, BEpiResid{b64.b, bi.b}, ost, false);
+            CM_TRY(b64.check_racc());
             CM_HIP(hipMemcpyAsync(host_ost, ost, KB * sizeof(SolveState), hipMemcpyDeviceToHost, stream));
             CM_HIP(hipStreamSynchronize(stream));
             bool any = false;
@@ -2256,9 +2306,14 @@ static size_t block_bytes_estimate(const K64 &k64, int kb, int bits)
         return s;
     };
     const size_t sl = slots(kb);
+    // long-row remainder sums of the matrices on the diagonal block products
+    size_t racc = 0;
+    for (const auto *m : {&k64.A, k64.sym ? &k64.GT : &k64.H, &k64.G})
+        if (m->block_dia && m->dia && m->dia->nlong)
+            racc = std::max(racc, (size_t)m->dia->nlong * kb * (m->dia->ntile > 1 ? 1 + (size_t)m->dia->ntile : 1));
     auto one = [&](size_t vsz) {
         return 11 * (size_t)n * kb * vsz + std::max(sl, (size_t)MAX_RED_BLOCKS) * 4 * kb * sizeof(double) +
-               3 * (size_t)n * sizeof(int) + kb * sizeof(SolveState) + sizeof(BlockFlags);
+               3 * (size_t)n * sizeof(int) + kb * sizeof(SolveState) + sizeof(BlockFlags) + racc * vsz;
     };
     size_t b = one(sizeof(double)) + (size_t)n * kb * sizeof(double);
     if (bits == 32 || bits == 16) b += one(sizeof(float));
@@ -2632,7 +2687,7 @@ struct BatchRunner {
             const int i = order[q];
             if (resval) resval[i] = hitems[q].resval;
             if (iter) iter[i] = hitems[q].iter;
-            if (info) info[i] = hitems[q].info;
+            if (info && hitems[q].info >= 0) info[i] = hitems[q].info;  // untouched when NMAX < 0
             if (cycles) cycles[i] = hitems[q].cycles;
         }
         return CM_OK;
@@ -2804,7 +2859,10 @@ int cm_system_block_dia(cm_system h, int on)
     S.block_dia = on != 0;
     for (auto &q : S.blks)  // captured block passes carry the old kernels
         if (q) q->drop_graphs();
-    return prepare_block_dia(S);
+    CM_TRY(prepare_block_dia(S));
+    for (auto &q : S.blks)  // a matrix may now need its long-row buffer
+        if (q) CM_TRY(q->ensure_workspace());
+    return CM_OK;
 }
 
 int cm_system_graphs(cm_system h, int on)

@@ -12,9 +12,13 @@
 
 namespace cm {
 
-// x[i] = (b[i] - sum_{j != i} T[i,j] x[j]) / T[i,i] for the rows of one level.
+// x[i] = (b[i] - sum_{j in triangle, j != i} T[i,j] x[j]) / T[i,i] for the
+// rows of one level.  Only the declared triangle is read (lower: j <= i,
+// upper: j >= i), as the TRIANGULAR descriptor makes mkl_sparse_d_trsv do
+// (ACIFIM/src/bicgstab.cpp:170-176); a stray entry on the other side is
+// ignored rather than read while another level writes it.
 template <class T, class V>
-__global__ __launch_bounds__(BS) void k_trsv_level(int cnt, const int *__restrict__ rows,
+__global__ __launch_bounds__(BS) void k_trsv_level(int cnt, bool lower, const int *__restrict__ rows,
                                                    const int *__restrict__ ptr, const int *__restrict__ col,
                                                    const V *__restrict__ val, const float *__restrict__ rowscale,
                                                    const T *__restrict__ b, T *__restrict__ x,
@@ -28,6 +32,7 @@ __global__ __launch_bounds__(BS) void k_trsv_level(int cnt, const int *__restric
     const double sc = rowscale ? (double)rowscale[i] : 1.0;
     for (int k = ptr[i]; k < ptr[i + 1]; k++) {
         const int j = col[k];
+        if (lower ? j > i : j < i) continue;
         const double a = load_val<V, double>(val, k) * sc;
         if (j == i) d = a;
         else s -= a * (double)x[j];

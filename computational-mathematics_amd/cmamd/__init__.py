@@ -543,10 +543,11 @@ class System:
         _check(self._lib.cm_system_block_get(self._h, _dp(X)))
         return X
 
-    def block_solve(self, tol=1e-8, nmax=500):
-        """FP64 block solve; per-column (resval, iter, info) arrays."""
+    def block_solve(self, tol=1e-8, nmax=500, info_in=0):
+        """FP64 block solve; per-column (resval, iter, info) arrays.  info
+        starts at info_in (the engine leaves it so when nmax < 0)."""
         k = self._bk
-        rv, it, info = np.empty(k), np.empty(k, np.int32), np.empty(k, np.int32)
+        rv, it, info = np.empty(k), np.empty(k, np.int32), np.full(k, info_in, np.int32)
         _check(self._lib.cm_system_block_solve(self._h, tol, nmax, _dp(rv), _ip(it), _ip(info)))
         return rv, it, info
 
@@ -742,10 +743,11 @@ class Server:
             pass
 
 
-def batch_solve(items, restart_true=False):
+def batch_solve(items, restart_true=False, info_in=0):
     """Persistent batched FP64 solves of small resident systems in one launch.
     items: list of (System, b, x0 or None, tol, nmax).  Returns a list of
-    SolveResult (x, resval, iter, info)."""
+    SolveResult (x, resval, iter, info); info starts at info_in (the engine
+    leaves it so for an item with nmax < 0)."""
     lib = solve_lib()
     m = len(items)
     hs = (C.c_void_p * m)(*[it[0]._h for it in items])
@@ -755,7 +757,7 @@ def batch_solve(items, restart_true=False):
     px = (PD * m)(*[_dp(x) for x in xs])
     tol = np.array([it[3] for it in items], np.float64)
     nmax = np.array([it[4] for it in items], np.int32)
-    rv, it_, info = np.empty(m), np.empty(m, np.int32), np.empty(m, np.int32)
+    rv, it_, info = np.empty(m), np.empty(m, np.int32), np.full(m, info_in, np.int32)
     _check(lib.cm_batch_solve(m, hs, pb, px, _dp(tol), _ip(nmax), 1 if restart_true else 0, _dp(rv), _ip(it_),
                               _ip(info)))
     return [SolveResult(xs[i], float(rv[i]), int(it_[i]), int(info[i])) for i in range(m)]

@@ -141,6 +141,29 @@ def test_block_long_rows():
     _check_columns("afiim", A, F, B, X, rv, it, info, 500)
 
 
+def test_block_dia_switched_on_after_block_set():
+    # the long-row buffer of the diagonal block products must exist when a
+    # block workspace made with them off is used after switching them on
+    A = cmamd.gen_stencil27(14, 0.05, 200, 600, seed=9)
+    F = cmamd.afiim(A, 0.1)
+    B = _rhs(A, 4, 60)
+    sys_ = cmamd.System("afiim", A, F)
+    sys_.block_dia(False)
+    sys_.block_set(B)
+    sys_.enable_mixed(32)
+    sys_.block_dia(True)
+    rv, it, info = sys_.block_solve(tol=1e-8, nmax=500)
+    X = sys_.block_get()
+    _check_columns("afiim", A, F, B, X, rv, it, info, 500)
+    sys_.block_set(None)
+    rvm, outer, inner, infom = sys_.block_solve_mixed(32, tol=1e-8)
+    Xm = sys_.block_get()
+    sys_.close()
+    assert np.all(infom == 0)
+    for j in range(B.shape[1]):
+        assert np.linalg.norm(B[:, j] - cmamd.csr_mv_host(A, Xm[:, j])) / np.linalg.norm(B[:, j]) <= 1e-8
+
+
 def test_block_pdfiim_symmetric():
     import scipy.sparse as sp
     A = cmamd.gen_convdiff(3, 14, 0.0, seed=13)

@@ -14,7 +14,10 @@ fix -- never against a retyped copy of its own formulas:
 * the exit codes against hand-derived cases (NMAX reached; NMAX <= 0; the
   unguarded omega = 0/0 breakdown of an exact preconditioner, where AFIIM and
   PDFIIM run to NMAX on NaNs and ACIFIM's finite-alpha guard stops at pass 1;
-  stagnation below rounding).
+  stagnation below rounding; divergence on systems whose singular
+  preconditioned operator makes the residual grow on every pass, with the exit
+  pass derived from the textbook trace and the reference's counting rule, for
+  the AFIIM / PDFIIM limit at three sizes and the ACIFIM limit).
 """
 import numpy as np
 import pytest
@@ -22,6 +25,7 @@ import scipy.linalg as sla
 import scipy.sparse as sp
 
 import cmamd
+import exit_cases
 import oracle
 
 
@@ -83,6 +87,25 @@ def _tri_factors(n, seed):
     idv = rng.uniform(0.5, 2.0, n)
     c = lambda M: cmamd.Csr.from_scipy(sp.csr_matrix(M))  # noqa: E731
     return cmamd.Factors(c(G), idv, c(H), c(L), c(U)), G, H, L, U, idv
+
+
+def _with_stray(T, lower, seed):
+    # the triangular factor plus entries on the wrong side of the diagonal,
+    # which the TRIANGULAR descriptor makes mkl_sparse_d_trsv ignore
+    rng = np.random.default_rng(seed)
+    n = T.shape[0]
+    S = (np.triu if lower else np.tril)(rng.uniform(-5, 5, (n, n)) * (rng.random((n, n)) < 0.05),
+                                         1 if lower else -1)
+    return cmamd.Csr.from_scipy(sp.csr_matrix(T + S))
+
+
+def test_trsv_ignores_entries_outside_the_triangle():
+    F, G, H, L, U, idv = _tri_factors(80, 3)
+    b = np.random.default_rng(4).uniform(-1, 1, 80)
+    Ls, Us = _with_stray(L, True, 5), _with_stray(U, False, 6)
+    assert Ls.nnz > F.L.nnz and Us.nnz > F.U.nnz
+    assert np.allclose(oracle.trsv(Ls, b, lower=True), sla.solve_triangular(L, b, lower=True), rtol=1e-13)
+    assert np.allclose(oracle.trsv(Us, b, lower=False), sla.solve_triangular(U, b, lower=False), rtol=1e-13)
 
 
 def test_trsv_against_scipy():
@@ -221,8 +244,41 @@ def test_exit_exact_preconditioner_breakdown():
 def test_exit_stagnation_below_rounding():
     A, b, F = _c1_small()
     x, rv, it, info, tr = oracle.bicgstab("afiim", A, b, F, tol=1e-30, nmax=500, trace_cap=500)
-    assert info in (1, 2)
+    assert info == 2
     assert it < 500 and rv < 1e-12
+
+
+@pytest.mark.parametrize("name", sorted(exit_cases.DIVERGING))
+@pytest.mark.parametrize("variant", ["afiim", "pdfiim"])
+def test_exit_divergence_afiim_limit(name, variant):
+    # n = 64, 600, 1024: maxmsteps 10, 12, 20 -- the exit pass follows from an
+    # independent textbook trace and the reference's counting rule
+    A, b, F, d = exit_cases.diverging(name)
+    limit = exit_cases.maxmsteps(variant, A.n)
+    assert limit == {"d64": 10, "d600": 12, "d1024": 20}[name]
+    norms = exit_cases.textbook_norms(d, b, 60)
+    it_e, rv_e = exit_cases.divergence_exit(norms, limit)
+    assert it_e == limit  # the residual grows from the first pass on
+    x, rv, it, info, tr = oracle.bicgstab(variant, A, b, F, tol=1e-8, nmax=200, trace_cap=100)
+    assert (it, info) == (it_e, 1)
+    assert abs(rv - rv_e) <= 1e-9 * rv_e
+    m = len(tr)
+    assert m == it + 1 and np.max(np.abs(tr - norms[:m]) / norms[:m]) < 1e-9
+
+
+def test_exit_divergence_acifim_limit():
+    # ACIFIM allows 40 growing passes (ACIFIM/src/bicgstab.cpp:132)
+    A, b, F, d = exit_cases.diverging("d600")
+    assert exit_cases.maxmsteps("acifim", A.n) == 40
+    x, rv, it, info, tr = oracle.bicgstab("acifim", A, b, F, tol=1e-8, nmax=200, trace_cap=100)
+    assert (it, info) == (40, 1)
+    assert exit_cases.divergence_exit(tr, 40)[0] == 40
+    assert exit_cases.divergence_exit(exit_cases.textbook_norms(d, b, 60), 40)[0] == 40
+    # past the AFIIM / PDFIIM limit of the same system without stopping
+    for name in ("d64", "d1024"):
+        A, b, F, d = exit_cases.diverging(name)
+        x, rv, it, info, tr = oracle.bicgstab("acifim", A, b, F, tol=1e-8, nmax=200)
+        assert it > exit_cases.maxmsteps("afiim", A.n)
 
 
 def test_thread_count_only_changes_rounding():

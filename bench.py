@@ -32,34 +32,7 @@ HBM_PEAK_GBS = 8000.0
 SAMPLE_PASSES = 8  # passes per timed solve whose dominant-kernel launches are bracketed by HIP events
 METRIC = "PBiCGSTAB solves/s + p99 latency at rel-res 1e-8; SpMV/M-apply HBM GB/s vs peak"
 
-WORKLOADS = {
-    "cd3d-128": dict(
-        desc="3D 7-point convection-diffusion 128^3 (n=2097152, nnz=14581760), log-normal diffusion, "
-             "velocity (1,1,1), cell Peclet 1; PDFIIM(prefill=2, lfill=2, postfill=1, eps=0.05); "
-             "b=A x*, x*~U[-1,1], x0=0; plain FP64 BiCGSTAB to rel-res 1e-8",
-        dim=3, N=128, pe=1.0, seed=2, xseed=3, prec="pdfiim", args=(2, 2, 1, 0.05), tol=1e-8, nmax=1200),
-    "cd3d-128-pe5": dict(
-        desc="3D 7-point convection-diffusion 128^3 (n=2097152, nnz=14581760), cell Peclet 5; "
-             "ACIFIM(eps1=0.01, eps2=0.1), apply mode E; b=A x*, x0=0; plain FP64 BiCGSTAB to rel-res 1e-10",
-        dim=3, N=128, pe=5.0, seed=4, xseed=5, prec="acifim", args=(0.01, 0.1), tol=1e-10, nmax=2000),
-    "st27-100-rhs16": dict(
-        desc="3D 27-point stencil 100^3 (n=1000000), off-diagonals -(1+0.3 U[-1,1]), diagonal 1.05 x row "
-             "absolute sum, 5% of rows with 200-600 long-range couplings -0.01 U[0,1] (nnz~46.3M); AFIIM "
-             "eps=0.05; block of 16 RHS b_j=A x*_j, x0=0; FP32 inner block BiCGSTAB (FP32 A and factors, "
-             "to 1e-4) inside per-column FP64 refinement to true rel-res 1e-8",
-        gen="stencil27", N=100, frac=0.05, lo=200, hi=600, seed=11, xseed=12, prec="afiim", args=(0.05,),
-        tol=1e-8, nmax=500, k=16),
-    "serve-mix": dict(
-        desc="serving: Poisson arrivals, 2D 5-point / 3D 7-point convection-diffusion systems (50/50), n "
-             "log-uniform in [1e3, 5e5], log-normal diffusion, Peclet U[0,5], AFIIM eps=0.1 resident; 1-8 RHS "
-             "per request; FP32-inner / FP64-refinement to true rel-res 1e-8 with continuous batching, vs the "
-             "plain FP64 one-at-a-time GPU path on the same requests",
-        prec="afiim", tol=1e-8, nmax=1000),
-    "cd2d-100": dict(
-        desc="2D 5-point convection-diffusion 100^2 (n=10000, nnz=49600), cell Peclet 0.5; AFIIM eps=0.1; "
-             "b=A x*, x0=0; plain FP64 BiCGSTAB to rel-res 1e-8",
-        dim=2, N=100, pe=0.5, seed=1, xseed=2, prec="afiim", args=(0.1,), tol=1e-8, nmax=500),
-}
+from cmamd.workloads import WORKLOADS, build as build_workload, serve_requests, serve_systems  # noqa: E402
 
 CAT_NAMES = {"A": "A SpMV + fused dot(s)", "F1": "factor SpMV H (D^-1 fused)", "F2": "factor SpMV G",
              "LU": "L/U SpMV (mode C)", "VEC": "BiCGSTAB vector updates", "SETUP": "setup r=b-Ax",
@@ -242,25 +215,6 @@ def trace_summary(db, thr_us=5.0, top=25):
     print("|---|---|---|---|---|---|")
     for t, n, c, sh, a, ne, ae in out[:top]:
         print(f"| `{n}` | {c} | {100 * sh:.1f}% | {a:.1f} | {ne} | {ae:.1f} |")
-
-
-def build_workload(w):
-    import cmamd
-    if w.get("gen") == "stencil27":
-        A = cmamd.gen_stencil27(w["N"], w["frac"], w["lo"], w["hi"], seed=w["seed"])
-    else:
-        A = cmamd.gen_convdiff(w["dim"], w["N"], w["pe"], seed=w["seed"])
-    xs = cmamd.gen_uniform(w["xseed"], A.n)
-    b = cmamd.csr_mv_host(A, xs)
-    t0 = time.perf_counter()
-    if w["prec"] == "pdfiim":
-        F = cmamd.pdfiim(A, *w["args"], -1, "I", "B")
-    elif w["prec"] == "afiim":
-        F = cmamd.afiim(A, *w["args"])
-    else:
-        F = cmamd.acifim(A, *w["args"])
-    t_build = time.perf_counter() - t0
-    return A, xs, b, F, t_build
 
 
 def cpu_baseline(w, A, b, F, seconds):
@@ -463,28 +417,6 @@ def bench_block(args, w):
     print(json.dumps(out), flush=True)
 
 
-def _serve_systems(nsys, seed):
-    """nsys systems: kind 2D/3D 50/50, n log-uniform in [1e3, 5e5], N =
-    round(n^(1/d)), Peclet U[0,5], log-normal diffusion; AFIIM eps=0.1 built
-    on host threads."""
-    import cmamd
-    from concurrent.futures import ThreadPoolExecutor
-    rng = np.random.default_rng(seed)
-    specs = []
-    for s in range(nsys):
-        d = 2 if rng.random() < 0.5 else 3
-        n = float(np.exp(rng.uniform(np.log(1e3), np.log(5e5))))
-        N = max(4, int(round(n ** (1.0 / d))))
-        specs.append((d, N, float(rng.uniform(0.0, 5.0)), 1000 + s))
-
-    def one(sp):
-        d, N, pe, sd = sp
-        A = cmamd.gen_convdiff(d, N, pe, seed=sd)
-        return A, cmamd.afiim(A, 0.1)
-    with ThreadPoolExecutor(max_workers=min(16, os.cpu_count() or 1)) as ex:
-        return list(ex.map(one, specs))
-
-
 def _progress(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
@@ -496,23 +428,11 @@ def bench_serve(args, w):
     arrival clock; latency = arrival to solution on the host."""
     import threading
     import cmamd
-    rng = np.random.default_rng(args.serve_seed)
     t0 = time.perf_counter()
-    systems = _serve_systems(args.serve_systems, args.serve_seed)
+    systems = serve_systems(args.serve_systems, args.serve_seed)
     t_build = time.perf_counter() - t0
-    # per system a pool of 8 right-hand sides b = A x*
-    pools = []
-    for s, (A, _) in enumerate(systems):
-        XS = np.stack([cmamd.gen_uniform(50000 + 8 * s + j, A.n) for j in range(8)], axis=1)
-        pools.append((XS, np.stack([cmamd.csr_mv_host(A, XS[:, j]) for j in range(8)], axis=1)))
-    gaps = rng.exponential(1.0 / args.serve_rate, args.serve_requests)
-    arrive = np.cumsum(gaps)
-    reqs = []
-    for i in range(args.serve_requests):
-        s = int(rng.integers(len(systems)))
-        k = int(rng.integers(1, 9))
-        cols = np.sort(rng.choice(8, size=k, replace=False))
-        reqs.append((s, cols))
+    # per system a pool of 8 right-hand sides b = A x*; Poisson arrivals
+    pools, arrive, reqs = serve_requests(systems, args.serve_requests, args.serve_rate, args.serve_seed)
     n_tot = sum(A.n for A, _ in systems)
     print(f"[bench] workload serve-mix: {len(systems)} systems (total n={n_tot}), built in {t_build:.1f}s; "
           f"{len(reqs)} requests at {args.serve_rate}/s", file=sys.stderr, flush=True)

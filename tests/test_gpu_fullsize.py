@@ -1,0 +1,180 @@
+"""Parity at the full sizes of the benchmark configurations (the C1 case, the
+2D 100^2 system, runs at its size in test_gpu_parity.py):
+
+* cd3d-128 -- 3D 7-point 128^3 (n = 2,097,152), Peclet 1, PDFIIM(2,2,1,0.05),
+  FP64 to 1e-8: per-pass residuals within 1e-8 relative over the first 20
+  passes (rounding floor 1e-12 ||r_0||), iterations within 2%, same info,
+  solution within 1e-6 of the oracle's, true relative residual <= 1e-8;
+  FP32 and FP16-factor inner solves inside FP64 refinement to true relative
+  residual 1e-8, within 1e-5 of the oracle's FP64 solution;
+* cd3d-128-pe5 -- Peclet 5, ACIFIM(0.01, 0.1), FP64 to 1e-10 with the same
+  bars, and FP32 / FP16-factor refinement to true relative residual 1e-10;
+* st27-100-rhs16 -- 27-point 100^3 with 5% long rows, AFIIM 0.05, 16
+  right-hand sides: FP64 block solve per column against the oracle (first
+  four columns) and every column's exit code and true residual; FP32-inner
+  block refinement to true relative residual 1e-8 on all 16 columns;
+* serve-mix -- 200 resident systems, 10,000 requests of 1-8 columns, run at
+  saturation (a window of outstanding requests instead of the 200/s arrival
+  clock, to bound the test's time): every column info 0 and true relative
+  residual <= 1e-8.
+
+Every system, factor and right-hand side is the benchmark's own
+(cmamd.workloads); every solve goes through the C ABI of libcmsolve.so.
+"""
+import os
+
+import numpy as np
+import pytest
+
+import cmamd
+import oracle
+from cmamd import workloads
+
+pytestmark = pytest.mark.gpu
+
+TRACE_TOL = 1e-8
+
+
+def _threads():
+    return max(1, min(16, int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or (os.cpu_count() or 1)))
+
+
+def _rel(a, b):
+    return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+
+def _true_res(S, b, x):
+    return float(np.linalg.norm(b - S @ x) / np.linalg.norm(b))
+
+
+def _trace_close(tr, tro, rel=TRACE_TOL, floor=1e-12):
+    return bool(np.all(np.abs(tr - tro) <= rel * tro + floor * tro[0]))
+
+
+def _single_workload(name, variant):
+    w = workloads.WORKLOADS[name]
+    print(f"full size {name}: building", flush=True)
+    A, xs, b, F, t_build = workloads.build(w)
+    S = A.to_scipy()
+    oracle.set_threads(_threads())
+    xo, rvo, ito, info_o, tro = oracle.bicgstab(w["prec"], A, b, F, tol=w["tol"], nmax=w["nmax"], trace_cap=64)
+    oracle.set_threads(1)
+    print(f"full size {name}: n={A.n} factors {t_build:.1f}s, oracle {ito} passes info {info_o}", flush=True)
+    sys_ = cmamd.System(variant, A, F)
+    sys_.enable_mixed(32)
+    sys_.enable_mixed(16)
+    sys_.autotune(reps=3)  # the benchmark's configuration
+    sys_.set_rhs(b)
+    sys_.set_x(None)
+    rv, it, info, tr = sys_.solve(tol=w["tol"], nmax=w["nmax"], trace_cap=64)
+    x = sys_.get_x()
+    m = min(20, len(tr), len(tro))
+    assert m == 20 and _trace_close(tr[:m], tro[:m])
+    assert info == info_o == 0
+    assert abs(it - ito) <= max(1, int(np.ceil(0.02 * ito)))
+    assert _rel(x, xo) <= 1e-6
+    assert _true_res(S, b, x) <= w["tol"]
+    for bits in (32, 16):
+        print(f"full size {name}: mixed {bits}", flush=True)
+        sys_.set_x(None)
+        rvm, outer, inner, infom = sys_.solve_mixed(bits, tol=w["tol"], inner_tol=1e-4, nmax=w["nmax"])
+        xm = sys_.get_x()
+        assert infom == 0, (bits, rvm, outer, inner)
+        assert _true_res(S, b, xm) <= w["tol"], bits
+        assert _rel(xm, xo) <= (1e-5 if w["tol"] >= 1e-8 else 1e-7), bits
+    sys_.close()
+
+
+@pytest.mark.timeout(600)
+def test_fullsize_cd3d_128_pdfiim():
+    _single_workload("cd3d-128", "pdfiim")
+
+
+@pytest.mark.timeout(600)
+def test_fullsize_cd3d_128_pe5_acifim():
+    _single_workload("cd3d-128-pe5", "acifim")
+
+
+@pytest.mark.timeout(600)
+def test_fullsize_st27_100_rhs16_block():
+    w = workloads.WORKLOADS["st27-100-rhs16"]
+    print("full size st27-100-rhs16: building", flush=True)
+    A, _, _, F, t_build = workloads.build(w)
+    k = w["k"]
+    XS = np.stack([cmamd.gen_uniform(w["xseed"] + j, A.n) for j in range(k)], axis=1)
+    B = np.stack([cmamd.csr_mv_host(A, XS[:, j]) for j in range(k)], axis=1)
+    S = A.to_scipy()
+    sys_ = cmamd.System("afiim", A, F)
+    sys_.enable_mixed(32)
+    sys_.block_set(B)
+    rv, it, info = sys_.block_solve(tol=w["tol"], nmax=w["nmax"])
+    X = sys_.block_get()
+    print(f"full size st27-100-rhs16: n={A.n} nnz={A.nnz} factors {t_build:.1f}s; FP64 block {list(it)}",
+          flush=True)
+    assert np.all(info == 0)
+    R = B - S @ X
+    # the FP64 loop stops on the recursive residual: the true one may sit
+    # above tol by the residual gap, which stays far below 1e-7 here
+    for j in range(k):
+        assert np.linalg.norm(R[:, j]) / np.linalg.norm(B[:, j]) <= 1e-7, j
+    oracle.set_threads(_threads())
+    for j in range(4):
+        xo, rvo, ito, info_o, _ = oracle.bicgstab("afiim", A, B[:, j], F, tol=w["tol"], nmax=w["nmax"])
+        assert info[j] == info_o, j
+        assert abs(it[j] - ito) <= max(1, int(np.ceil(0.02 * ito))), (j, it[j], ito)
+        assert _rel(X[:, j], xo) <= 1e-6, j
+    oracle.set_threads(1)
+    sys_.block_set(None)
+    rvm, outer, inner, infom = sys_.block_solve_mixed(32, tol=w["tol"], inner_tol=1e-4, nmax=w["nmax"])
+    Xm = sys_.block_get()
+    sys_.close()
+    assert np.all(infom == 0), infom
+    R = B - S @ Xm
+    for j in range(k):
+        assert np.linalg.norm(R[:, j]) / np.linalg.norm(B[:, j]) <= w["tol"], j
+        assert _rel(Xm[:, j], X[:, j]) <= 1e-5, j
+
+
+@pytest.mark.timeout(900)
+def test_fullsize_serve_mix_saturated():
+    w = workloads.WORKLOADS["serve-mix"]
+    print("full size serve-mix: building 200 systems", flush=True)
+    systems = workloads.serve_systems(200, 5)
+    pools, _, reqs = workloads.serve_requests(systems, 10000, 200.0, 5)
+    mats = [A.to_scipy() for A, _ in systems]
+    handles = [cmamd.System("afiim", A, F) for A, F in systems]
+    srv = cmamd.Server(slots=4, precision=32, small_n=8192)
+    sid = [srv.add_system(h) for h in handles]
+    print(f"full size serve-mix: {len(handles)} systems registered; {len(reqs)} requests", flush=True)
+    window = 256
+    outstanding = []
+    worst, columns, bad = 0.0, 0, []
+
+    def finish(i, rid):
+        nonlocal worst, columns
+        r = srv.wait(rid)
+        s, cols = reqs[i]
+        Bq = pools[s][1][:, cols]
+        res = np.linalg.norm(Bq - mats[s] @ r.X, axis=0) / np.linalg.norm(Bq, axis=0)
+        worst = max(worst, float(res.max()))
+        columns += len(cols)
+        if np.any(r.info != 0) or np.any(res > w["tol"]):
+            bad.append((i, s, [int(c) for c in r.info], [float(v) for v in res]))
+
+    for i, (s, cols) in enumerate(reqs):
+        outstanding.append((i, srv.submit(sid[s], pools[s][1][:, cols], tol=w["tol"], nmax=w["nmax"])))
+        if len(outstanding) >= window:
+            finish(*outstanding.pop(0))
+        if i and i % 2000 == 0:
+            print(f"full size serve-mix: {i} requests submitted", flush=True)
+    while outstanding:
+        finish(*outstanding.pop(0))
+    st = srv.stats()
+    srv.close()
+    for h in handles:
+        h.close()
+    print(f"full size serve-mix: {columns} columns, worst true residual {worst:.2e}, "
+          f"{st['persistent_columns']} through the persistent kernel", flush=True)
+    assert not bad, bad[:5]
+    assert columns == sum(len(c) for _, c in reqs)
+    assert st["persistent_columns"] > 0 and st["batches"] > 0

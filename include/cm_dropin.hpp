@@ -13,8 +13,9 @@
 //   ACIFIM: PBiCGSTAB(n, Av, Aj, Ai, b, tol, NMAX, Gv, Gj, Gi, IDv, Hv, Hj, Hi,
 //                     Lv, Lj, Li, Uv, Uj, Ui, nr, mode, x, &resval, &iter,
 //                     verbose, &info)  (ACIFIM/src/bicgstab.hpp:94-120)
-// and the factor producers afiim() / acifim() with the reference signatures
-// (AFIIM/src/prec.hpp:99-114, ACIFIM/src/prec.hpp:126-151).
+// and the factor producers afiim() / acifim() / pdfiim() with the reference
+// signatures (AFIIM/src/prec.hpp:99-114, ACIFIM/src/prec.hpp:126-151,
+// PDFIIM/src/parPrec.hpp:86-96).
 //
 // The reference functions return void and never fail short of std::bad_alloc;
 // an engine error (no GPU, HIP failure, bad CSR) is raised here as
@@ -79,6 +80,39 @@ inline int acifim(int n, double *Av, int *Aj, int *Ai, double **Gval, int **Gind
 {
     return cm_acifim(n, Av, Aj, Ai, Gval, Gindj, Gindi, IDv, Hval, Hindj, Hindi, Lval, Lindj, Lindi, Uval, Uindj,
                      Uindi, dtol1, dtol2, elemPerRowCol, growth, eta, shift, coeff, element_limit, filtit);
+}
+
+// PDFIIM factors into sparseMatrix-like G (CSR, UPPERTRI) and H (CSR,
+// LOWERTRI) and the inverse diagonal D, from A's stored CSR arrays, as the
+// reference's pdfiim() fills them (PDFIIM/src/parPrec.cpp:536-587).  Every
+// host thread is used (the reference's OpenMP default).
+template <class SparseMatrix>
+inline void pdfiim(SparseMatrix &A, SparseMatrix &G, std::vector<double> &D, SparseMatrix &H, int prefill,
+                   int lfill, int postfill, double ftol, int warpsize, char filttype, char patt)
+{
+    const int n = (int)A.r;
+    double *Gv = nullptr, *Hv = nullptr, *IDv = nullptr;
+    int *Gi = nullptr, *Gj = nullptr, *Hi = nullptr, *Hj = nullptr;
+    if (cm_pdfiim(n, A.v.data(), A.j.data(), A.i.data(), &Gv, &Gj, &Gi, &IDv, &Hv, &Hj, &Hi, prefill, lfill,
+                  postfill, ftol, warpsize, filttype, patt, 0) != 0)
+        throw std::runtime_error("cm pdfiim: unsupported arguments (patt 'L'/'U' need lfill == 1)");
+    auto fill = [n](SparseMatrix &M, int mtype, double *v, int *j, int *i) {
+        M.stype = 0;  // CSR (PDFIIM/src/constants.hpp:6)
+        M.mtype = mtype;
+        M.r = n;
+        M.c = n;
+        M.nnz = i[n];
+        M.i.assign(i, i + n + 1);
+        M.j.assign(j, j + i[n]);
+        M.v.assign(v, v + i[n]);
+        cm_free_d(v);
+        cm_free_i(j);
+        cm_free_i(i);
+    };
+    fill(G, 2, Gv, Gj, Gi);  // UPPERTRI
+    fill(H, 3, Hv, Hj, Hi);  // LOWERTRI
+    D.assign(IDv, IDv + n);
+    cm_free_d(IDv);
 }
 
 #endif

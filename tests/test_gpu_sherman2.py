@@ -4,8 +4,9 @@ committed as tests/golden/sherman2.npz, with the reference-built factors for
 the README parameters (tests/golden/afiim_sherman2.npz, acifim_sherman2.npz).
 
 Every solve variant and apply mode runs through the C ABI and is compared with
-the oracle on the same factors; the drop-in example driver (reference call
-signature via include/cm_dropin.hpp) is run end to end.
+the oracle on the same factors; the drop-in example drivers (the reference
+call signatures of all three packages via include/cm_dropin.hpp, compiled
+C++) are run end to end.
 
 sherman2 has a condition number near 1e12, and its BiCGSTAB iterates amplify
 rounding: the oracle against itself with 1, 2 and 8 threads (only the
@@ -117,6 +118,95 @@ def test_dropin_example_binary(tmp_path):
     x = cmamd.read_mtx_vector(str(px))
     xo = oracle.bicgstab("afiim", A, b, _golden_afiim(), nmax=500)[0]
     assert np.linalg.norm(x - xo) / np.linalg.norm(xo) < 1e-4
+
+
+def _write_mtx(path, A, symmetric=False):
+    # coordinate file; a symmetric one stores the lower triangle, as Matrix
+    # Market symmetric files do
+    rows = []
+    for i in range(A.n):
+        for k in range(A.ptr[i], A.ptr[i + 1]):
+            j = int(A.idx[k])
+            if symmetric and j > i:
+                continue
+            rows.append("%d %d %.17g\n" % (i + 1, j + 1, A.val[k]))
+    with open(path, "w") as f:
+        f.write("%%%%MatrixMarket matrix coordinate real %s\n" % ("symmetric" if symmetric else "general"))
+        f.write(f"{A.n} {A.n} {len(rows)}\n")
+        f.writelines(rows)
+
+
+def _write_vec(path, b):
+    with open(path, "w") as f:
+        f.write(f"%%MatrixMarket matrix array real general\n{len(b)} 1\n")
+        for v in b:
+            f.write("%.17g\n" % v)
+
+
+def _driver(name, args):
+    exe = os.path.join(ROOT, "computational-mathematics_amd", "bin", name)
+    assert os.path.exists(exe), "example driver not built"
+    print(f"drop-in driver {name} {' '.join(args[2:])}", flush=True)
+    r = subprocess.run([exe] + args, capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    return r.stdout
+
+
+@pytest.mark.parametrize("mode,nr", [("C", 2), ("E", 1)])
+def test_dropin_acifim_binary(tmp_path, mode, nr):
+    # the reference's ACIFIM README invocation (ACIFIM/README:31: 0.01 0.01 2
+    # C) and mode E, through PBiCGSTAB's reference signature in compiled C++;
+    # the driver's factors are the reference's (bit-exact producer), so the
+    # oracle runs on the committed reference-built ones
+    A, b = _sherman2()
+    pa, pb, px = tmp_path / "A.mtx", tmp_path / "b.mtx", tmp_path / "x.mtx"
+    _write_mtx(pa, A)
+    _write_vec(pb, b)
+    _driver("acifim_solve", [str(pa), str(pb), "0.01", "0.01", str(nr), mode, str(px)])
+    x = cmamd.read_mtx_vector(str(px))
+    xo, rvo, ito, info_o, _ = oracle.bicgstab("acifim", A, b, _golden_acifim(), nmax=2000, nr=nr, mode=mode)
+    assert info_o == 0
+    assert np.linalg.norm(x - xo) / np.linalg.norm(xo) < 1e-4
+    assert np.linalg.norm(b - cmamd.csr_mv_host(A, x)) / np.linalg.norm(b) < 1e-7
+
+
+def test_dropin_pdfiim_binary_general(tmp_path):
+    # sparseMatrix-container EPBiCGSTAB and pdfiim() in compiled C++ on the
+    # reference's fixture (mtype GENERAL)
+    A, b = _sherman2()
+    pa, pb, px = tmp_path / "A.mtx", tmp_path / "b.mtx", tmp_path / "x.mtx"
+    _write_mtx(pa, A)
+    _write_vec(pb, b)
+    out = _driver("pdfiim_solve", [str(pa), str(pb), "2", "2", "1", "0.001", "-1", str(px)])
+    assert "symmetric 0" in out
+    x = cmamd.read_mtx_vector(str(px))
+    F = cmamd.pdfiim(A, 2, 2, 1, 0.001, -1, "I", "B")
+    xo, rvo, ito, info_o, _ = oracle.bicgstab("pdfiim", A, b, F, nmax=1200)
+    assert info_o == 0
+    assert np.linalg.norm(x - xo) / np.linalg.norm(xo) < 1e-4
+
+
+def test_dropin_pdfiim_binary_symmetric(tmp_path):
+    # a symmetric file (3D Pe = 0 convection-diffusion, lower triangle stored):
+    # read as its upper triangle with mtype SYMMETRIC, PDFIIM built on the
+    # stored triangle, solved on the symmetric branch M = G D^{-1} G^T with
+    # b = (stored entries) * 1, all as the reference driver does
+    Afull = cmamd.gen_convdiff(3, 12, 0.0, seed=21)
+    pa, px = tmp_path / "A.mtx", tmp_path / "x.mtx"
+    _write_mtx(pa, Afull, symmetric=True)
+    out = _driver("pdfiim_solve", [str(pa), "none", "2", "2", "1", "0.05", "-1", str(px)])
+    assert "symmetric 1" in out
+    x = cmamd.read_mtx_vector(str(px))
+    Up, sym = cmamd.read_mtx(str(pa), symmetric_mode=1)
+    assert sym and np.all(Up.idx >= np.repeat(np.arange(Up.n), np.diff(Up.ptr)))
+    b = cmamd.csr_mv_host(Up, np.ones(Up.n))
+    F = cmamd.pdfiim(Up, 2, 2, 1, 0.05, -1, "I", "B")
+    Fs = cmamd.Factors(F.G, F.idv, None)
+    xo, rvo, ito, info_o, _ = oracle.bicgstab("pdfiim", Up, b, Fs, nmax=1200, issym=True)
+    assert info_o == 0
+    assert np.linalg.norm(x - xo) / np.linalg.norm(xo) < 1e-6
+    S = Afull.to_scipy()
+    assert np.linalg.norm(b - S @ x) / np.linalg.norm(b) < 1e-7
 
 
 def test_block_and_batch_on_sherman2():

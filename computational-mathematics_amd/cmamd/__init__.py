@@ -47,6 +47,8 @@ _libs = {}
 def _load(name):
     if name not in _libs:
         path = os.path.join(LIBDIR, name)
+        if name == "libcmhost.so" and os.environ.get("CMAMD_HOST_LIB"):
+            path = os.environ["CMAMD_HOST_LIB"]  # e.g. the sanitized build, lib/asan/libcmhost.so
         if not os.path.exists(path):
             raise RuntimeError(f"{path} is not built: run __graft_entry__.build() "
                                f"or make -C computational-mathematics_amd")

@@ -24,7 +24,6 @@
 
 namespace cm {
 
-constexpr int SHORT_MAX = 64;  // longer rows go to the wave-per-row bin
 
 struct BlockFlags {
     int all_done;

@@ -113,45 +113,6 @@ __global__ __launch_bounds__(BS) void k_vmul(int n, const T *__restrict__ a, con
 
 // ------------------------------------------------------------ host helpers
 
-// Row blocks for the adaptive SpMV: consecutive rows with at most `tile`
-// nonzeros and ROWCAP rows, or a single longer row.
-static std::vector<int> make_row_blocks(int n, const int *ptr, int tile = TILE)
-{
-    std::vector<int> b;
-    b.push_back(0);
-    int rows = 0;
-    long long nz = 0;
-    for (int r = 0; r < n; r++) {
-        const int len = ptr[r + 1] - ptr[r];
-        if (rows > 0 && (nz + len > tile || rows >= ROWCAP)) {
-            b.push_back(r);
-            rows = 0;
-            nz = 0;
-        }
-        rows++;
-        nz += len;
-        if (len > tile) {
-            b.push_back(r + 1);
-            rows = 0;
-            nz = 0;
-        }
-    }
-    if (rows > 0) b.push_back(n);
-    return b;
-}
-
-// The SpMV kernels' row-block table: the block row starts, then the offsets
-// of the blocks' first nonzeros (ptr at those rows); 2 (nblk + 1) entries.
-static std::vector<int> block_table(int n, const int *ptr, int *nblk, int tile = TILE)
-{
-    std::vector<int> b = make_row_blocks(n, ptr, tile);
-    const size_t nb1 = b.size();
-    *nblk = (int)nb1 - 1;
-    b.resize(2 * nb1);
-    for (size_t i = 0; i < nb1; i++) b[nb1 + i] = ptr[b[i]];
-    return b;
-}
-
 template <class T>
 static int dev_alloc(T **d, size_t count)
 {
@@ -262,11 +223,6 @@ struct DiaIndex {
 // or 16-bit column indices, the slot-major blocks with either, or the
 // diagonals with a CSR remainder.
 enum SpmvFormat { FMT_STAGED = 0, FMT_STAGED16 = 1, FMT_SLOT = 2, FMT_SLOT16 = 3, FMT_DIA = 4, FMT_DIA2 = 5 };
-constexpr double DIA_FILL = 0.6;  // a diagonal is stored densely when at least this share of rows has it
-// Column tiles of the long remainders of the diagonal block products: about
-// one tile per LONG_TILE_COLS columns (2 MB of x rows at 16 FP32 columns),
-// at most LONG_TILES_MAX.
-constexpr int LONG_TILE_COLS = 32768, LONG_TILES_MAX = 32, LONG_SEG_MIN = 8;
 
 // Rows per thread of spmv_dia: FP16 values are read two rows at a time (one
 // 4-byte load per diagonal); FP32 and FP64 one row per thread.  The second
@@ -356,11 +312,7 @@ static int check_csr(int n, const int *ptr, const int *col)
 {
     // The kernels index x by the stored columns and stage rows by ptr: reject
     // anything that would read outside the arrays.
-    if (!ptr || !col || ptr[0] != 0) return fail(CM_ERR_ARG, "CSR row pointer must start at 0");
-    for (int i = 0; i < n; i++)
-        if (ptr[i + 1] < ptr[i]) return fail(CM_ERR_ARG, "CSR row pointer not monotone");
-    for (long long k = 0; k < ptr[n]; k++)
-        if (col[k] < 0 || col[k] >= n) return fail(CM_ERR_ARG, "CSR column index out of range");
+    if (const char *why = csr_problem(n, ptr, col)) return fail(CM_ERR_ARG, why);
     return CM_OK;
 }
 
@@ -465,62 +417,21 @@ static int build_slots(DevCsr<double> &m, hipStream_t s)
     CM_HIP(hipMemcpyAsync(ptr.data(), m.ptr, ptr.size() * sizeof(int), hipMemcpyDeviceToHost, s));
     if (m.nnz) CM_HIP(hipMemcpyAsync(col.data(), m.col, col.size() * sizeof(int), hipMemcpyDeviceToHost, s));
     CM_HIP(hipStreamSynchronize(s));
-    for (int i = 0; i < n; i++)
-        if (ptr[i + 1] - ptr[i] > SLOT_MAXLEN) return CM_OK;
-    const int nb = (n + SLOT_ROWS - 1) / SLOT_ROWS;
-    std::vector<int> hdr((size_t)nb * 5), soff, lrow(n), scol(m.nnz), src(m.nnz), cb(nb, 0);
-    std::vector<uint16_t> sd(m.nnz);
-    bool fits16 = true;
-    long long pos = 0;
-    std::vector<int> ord(SLOT_ROWS);
-    for (int b = 0; b < nb; b++) {
-        const int r0 = b * SLOT_ROWS, nr = std::min(SLOT_ROWS, n - r0);
-        auto len = [&](int r) { return ptr[r + 1] - ptr[r]; };
-        for (int i = 0; i < nr; i++) ord[i] = i;
-        std::stable_sort(ord.begin(), ord.begin() + nr, [&](int a, int c) { return len(r0 + a) > len(r0 + c); });
-        const int L = nr ? len(r0 + ord[0]) : 0;
-        int lo = INT32_MAX, hi = -1;
-        for (long long k = ptr[r0]; k < ptr[r0 + nr]; k++) {
-            lo = std::min(lo, col[k]);
-            hi = std::max(hi, col[k]);
-        }
-        if (hi >= 0 && hi - lo > 65535) fits16 = false;
-        cb[b] = hi >= 0 ? lo : 0;
-        hdr[5 * (size_t)b + 0] = r0;
-        hdr[5 * (size_t)b + 1] = nr;
-        hdr[5 * (size_t)b + 2] = (int)pos;
-        hdr[5 * (size_t)b + 3] = (int)soff.size();
-        hdr[5 * (size_t)b + 4] = L;
-        int off = 0;
-        for (int j = 0; j < L; j++) {
-            soff.push_back(off);
-            for (int i = 0; i < nr; i++) {
-                const int r = r0 + ord[i];
-                if (len(r) <= j) break;  // rows are in decreasing length
-                const long long k = ptr[r] + j;
-                scol[pos] = col[k];
-                src[pos] = (int)k;
-                sd[pos] = (uint16_t)(col[k] - cb[b]);
-                pos++;
-                off++;
-            }
-        }
-        soff.push_back(off);
-        for (int i = 0; i < nr; i++) lrow[r0 + i] = (len(r0 + ord[i]) << 16) | ord[i];
-    }
+    SlotLayout L;
+    if (!slot_layout(n, ptr.data(), col.data(), L)) return CM_OK;
     auto S = std::make_shared<SlotIndex>();
     S->n = n;
-    S->nb = nb;
+    S->nb = L.nb;
     S->nnz = m.nnz;
-    S->nsoff = soff.size();
-    CM_TRY(dev_upload(&S->hdr, hdr.data(), hdr.size(), s));
-    CM_TRY(dev_upload(&S->soff, soff.data(), soff.size(), s));
-    CM_TRY(dev_upload(&S->lrow, lrow.data(), lrow.size(), s));
-    CM_TRY(dev_upload(&S->col, scol.data(), scol.size(), s));
-    CM_TRY(dev_upload(&S->src, src.data(), src.size(), s));
-    if (fits16) {
-        CM_TRY(dev_upload(&S->d16, sd.data(), sd.size(), s));
-        CM_TRY(dev_upload(&S->cbase, cb.data(), cb.size(), s));
+    S->nsoff = L.soff.size();
+    CM_TRY(dev_upload(&S->hdr, L.hdr.data(), L.hdr.size(), s));
+    CM_TRY(dev_upload(&S->soff, L.soff.data(), L.soff.size(), s));
+    CM_TRY(dev_upload(&S->lrow, L.lrow.data(), L.lrow.size(), s));
+    CM_TRY(dev_upload(&S->col, L.col.data(), L.col.size(), s));
+    CM_TRY(dev_upload(&S->src, L.src.data(), L.src.size(), s));
+    if (L.fits16) {
+        CM_TRY(dev_upload(&S->d16, L.d16.data(), L.d16.size(), s));
+        CM_TRY(dev_upload(&S->cbase, L.cbase.data(), L.cbase.size(), s));
     }
     CM_HIP(hipStreamSynchronize(s));  // host temporaries
     m.slots = S;
@@ -551,100 +462,34 @@ static int build_dia(DevCsr<double> &m, hipStream_t s)
     CM_HIP(hipMemcpyAsync(ptr.data(), m.ptr, ptr.size() * sizeof(int), hipMemcpyDeviceToHost, s));
     if (m.nnz) CM_HIP(hipMemcpyAsync(col.data(), m.col, col.size() * sizeof(int), hipMemcpyDeviceToHost, s));
     CM_HIP(hipStreamSynchronize(s));
-    // entries per diagonal, indexed by offset + n - 1 (offsets span 1 - n .. n - 1)
-    std::vector<int> cnt(2 * (size_t)n - 1, 0);
-    for (int i = 0; i < n; i++)
-        for (int k = ptr[i]; k < ptr[i + 1]; k++) cnt[(size_t)(col[k] - i + n - 1)]++;
-    std::vector<std::pair<long long, int>> cand;
-    for (size_t o = 0; o < cnt.size(); o++)
-        if ((double)cnt[o] >= DIA_FILL * n) cand.push_back({cnt[o], (int)((long long)o - (n - 1))});
-    if (cand.empty()) return CM_OK;
-    std::sort(cand.begin(), cand.end(), [](const auto &a, const auto &b) {
-        return a.first != b.first ? a.first > b.first : a.second < b.second;
-    });
-    if ((int)cand.size() > DIA_MAX) cand.resize(DIA_MAX);
-    std::vector<int> off;
-    for (const auto &c : cand) off.push_back(c.second);
-    std::sort(off.begin(), off.end());
-    const int nd = (int)off.size();
-    const int ldn = (n + DIA_ALIGN - 1) / DIA_ALIGN * DIA_ALIGN;
-    // diagonal index by offset (reusing the count array): -1 for the others
-    std::fill(cnt.begin(), cnt.end(), -1);
-    for (int d = 0; d < nd; d++) cnt[(size_t)(off[d] + n - 1)] = d;
-    std::vector<int> dsrc((size_t)nd * ldn, -1), rptr(n + 1, 0), rcol, rsrc;
-    for (int i = 0; i < n; i++) {
-        for (int k = ptr[i]; k < ptr[i + 1]; k++) {
-            const int d = cnt[(size_t)(col[k] - i + n - 1)];
-            if (d >= 0) {
-                dsrc[(size_t)d * ldn + i] = k;
-            } else {
-                rcol.push_back(col[k]);
-                rsrc.push_back(k);
-            }
-        }
-        rptr[i + 1] = (int)rcol.size();
-    }
-    // long remainders (block products): the rows with more than SHORT_MAX
-    // remainder entries, as a tile-major CSR of (column tile, row) segments
-    // (counting sort by tile; entries keep their row order in a segment)
-    std::vector<int> lpos(n, -1), lrow;
-    for (int i = 0; i < n; i++)
-        if (rptr[i + 1] - rptr[i] > SHORT_MAX) {
-            lpos[i] = (int)lrow.size();
-            lrow.push_back(i);
-        }
-    const int nlong = (int)lrow.size();
-    long long nlong_rem = 0;
-    for (int q = 0; q < nlong; q++) nlong_rem += rptr[lrow[q] + 1] - rptr[lrow[q]];
-    int ntile = std::min(LONG_TILES_MAX, std::max(1, (n + LONG_TILE_COLS - 1) / LONG_TILE_COLS));
-    // at least LONG_SEG_MIN entries per segment on average: bounds the
-    // segment sums (written and read back by every block product, and held
-    // in the block workspace) by the long remainders' own size
-    if (nlong) ntile = (int)std::min<long long>(ntile, std::max<long long>(1, nlong_rem / ((long long)LONG_SEG_MIN * nlong)));
-    if ((long long)ntile * nlong > INT_MAX) ntile = 1;  // segment ids are int
-    const int tw = (n + ntile - 1) / ntile;
-    const long long nseg = (long long)ntile * nlong;
-    std::vector<int> lptr((size_t)nseg + 1, 0), lcol, lsrc;
-    for (int q = 0; q < nlong; q++)
-        for (int k = rptr[lrow[q]]; k < rptr[lrow[q] + 1]; k++) lptr[(size_t)(rcol[k] / tw) * nlong + q + 1]++;
-    for (long long g = 0; g < nseg; g++) lptr[g + 1] += lptr[g];
-    lcol.resize(lptr[nseg]);
-    lsrc.resize(lptr[nseg]);
-    {
-        std::vector<int> fill(lptr.begin(), lptr.end() - 1);
-        for (int q = 0; q < nlong; q++)
-            for (int k = rptr[lrow[q]]; k < rptr[lrow[q] + 1]; k++) {
-                const int pos = fill[(size_t)(rcol[k] / tw) * nlong + q]++;
-                lcol[pos] = rcol[k];
-                lsrc[pos] = rsrc[k];
-            }
-    }
+    DiaLayout L;
+    if (!dia_layout(n, ptr.data(), col.data(), L)) return CM_OK;
     auto D = std::make_shared<DiaIndex>();
     D->n = n;
-    D->ldn = ldn;
-    D->nd = nd;
-    D->nrem = (long long)rcol.size();
-    D->nlong = nlong;
-    D->ntile = ntile;
-    D->tw = tw;
-    D->nseg = nseg;
-    D->nlrem = (long long)lcol.size();
+    D->ldn = L.ldn;
+    D->nd = L.nd;
+    D->nrem = (long long)L.rcol.size();
+    D->nlong = L.nlong;
+    D->ntile = L.ntile;
+    D->tw = L.tw;
+    D->nseg = L.nseg;
+    D->nlrem = (long long)L.lcol.size();
     if (D->nlong) {
-        std::vector<int> iota((size_t)nseg);
-        for (long long g = 0; g < nseg; g++) iota[g] = (int)g;
-        CM_TRY(dev_upload(&D->lpos, lpos.data(), lpos.size(), s));
-        CM_TRY(dev_upload(&D->lptr, lptr.data(), lptr.size(), s));
-        CM_TRY(dev_upload(&D->lcol, lcol.data(), lcol.size(), s));
-        CM_TRY(dev_upload(&D->lsrc, lsrc.data(), lsrc.size(), s));
+        std::vector<int> iota((size_t)L.nseg);
+        for (long long g = 0; g < L.nseg; g++) iota[g] = (int)g;
+        CM_TRY(dev_upload(&D->lpos, L.lpos.data(), L.lpos.size(), s));
+        CM_TRY(dev_upload(&D->lptr, L.lptr.data(), L.lptr.size(), s));
+        CM_TRY(dev_upload(&D->lcol, L.lcol.data(), L.lcol.size(), s));
+        CM_TRY(dev_upload(&D->lsrc, L.lsrc.data(), L.lsrc.size(), s));
         CM_TRY(dev_upload(&D->lrows, iota.data(), iota.size(), s));
         CM_HIP(hipStreamSynchronize(s));  // host temporaries
     }
-    CM_TRY(dev_upload(&D->off, off.data(), off.size(), s));
-    CM_TRY(dev_upload(&D->dsrc, dsrc.data(), dsrc.size(), s));
+    CM_TRY(dev_upload(&D->off, L.off.data(), L.off.size(), s));
+    CM_TRY(dev_upload(&D->dsrc, L.dsrc.data(), L.dsrc.size(), s));
     if (D->nrem) {
-        CM_TRY(dev_upload(&D->rptr, rptr.data(), rptr.size(), s));
-        CM_TRY(dev_upload(&D->rcol, rcol.data(), rcol.size(), s));
-        CM_TRY(dev_upload(&D->rsrc, rsrc.data(), rsrc.size(), s));
+        CM_TRY(dev_upload(&D->rptr, L.rptr.data(), L.rptr.size(), s));
+        CM_TRY(dev_upload(&D->rcol, L.rcol.data(), L.rcol.size(), s));
+        CM_TRY(dev_upload(&D->rsrc, L.rsrc.data(), L.rsrc.size(), s));
     }
     CM_HIP(hipStreamSynchronize(s));  // host temporaries
     m.dia = D;
@@ -670,88 +515,17 @@ static int build_dia_vals(DevCsr<V> &m, hipStream_t s)
     return CM_OK;
 }
 
-// Dependency levels of a unit triangular factor: lower -> a row reads only
-// smaller columns (rows in increasing order); upper -> larger columns.
+// Dependency levels of a triangular factor (level_schedule): one launch of
+// the triangular-solve kernel per level.
 static int build_levels(DevCsr<double> &m, const int *ptr, const int *col, bool lower, hipStream_t s)
 {
-    const int n = m.n;
-    std::vector<int> lev(n, 0);
-    int nlev = 0;
-    for (int q = 0; q < n; q++) {
-        const int i = lower ? q : n - 1 - q;
-        int l = 0;
-        for (int k = ptr[i]; k < ptr[i + 1]; k++) {
-            const int j = col[k];
-            if ((lower && j < i) || (!lower && j > i)) l = std::max(l, lev[j] + 1);
-        }
-        lev[i] = l;
-        nlev = std::max(nlev, l + 1);
-    }
-    m.nlev = nlev;
+    LevelSchedule L = level_schedule(m.n, ptr, col, lower);
+    m.nlev = L.nlev;
     m.lower = lower;
-    m.loff.assign(nlev + 1, 0);
-    for (int i = 0; i < n; i++) m.loff[lev[i] + 1]++;
-    for (int l = 0; l < nlev; l++) m.loff[l + 1] += m.loff[l];
-    std::vector<int> rows(n), fill(m.loff.begin(), m.loff.end() - 1);
-    for (int i = 0; i < n; i++) rows[fill[lev[i]]++] = i;
-    CM_TRY(dev_upload(&m.lrows, rows.data(), (size_t)n, s));
+    m.loff = L.loff;
+    CM_TRY(dev_upload(&m.lrows, L.rows.data(), (size_t)m.n, s));
     CM_HIP(hipStreamSynchronize(s));
     return CM_OK;
-}
-
-// CSR of the transpose (host).
-static void host_transpose(int n, const int *ptr, const int *col, const double *val, std::vector<int> &tp,
-                           std::vector<int> &tc, std::vector<double> &tv)
-{
-    tp.assign(n + 1, 0);
-    for (int k = 0; k < ptr[n]; k++) tp[col[k] + 1]++;
-    for (int i = 0; i < n; i++) tp[i + 1] += tp[i];
-    tc.resize(ptr[n]);
-    tv.resize(ptr[n]);
-    std::vector<int> fill(tp.begin(), tp.end() - 1);
-    for (int i = 0; i < n; i++)
-        for (int k = ptr[i]; k < ptr[i + 1]; k++) {
-            int q = fill[col[k]]++;
-            tc[q] = i;
-            tv[q] = val[k];
-        }
-}
-
-// Full CSR of a symmetric matrix given its upper triangle (entries below the
-// diagonal, if any, are ignored as the SYMMETRIC/UPPER descriptor does).
-static void host_sym_expand(int n, const int *ptr, const int *col, const double *val, std::vector<int> &fp,
-                            std::vector<int> &fc, std::vector<double> &fv)
-{
-    fp.assign(n + 1, 0);
-    for (int i = 0; i < n; i++)
-        for (int k = ptr[i]; k < ptr[i + 1]; k++) {
-            const int j = col[k];
-            if (j < i) continue;
-            fp[i + 1]++;
-            if (j != i) fp[j + 1]++;
-        }
-    for (int i = 0; i < n; i++) fp[i + 1] += fp[i];
-    fc.resize(fp[n]);
-    fv.resize(fp[n]);
-    std::vector<int> fill(fp.begin(), fp.end() - 1);
-    // lower part first (column order ascending because rows are scanned in
-    // order), then the stored upper part of the row
-    for (int i = 0; i < n; i++)
-        for (int k = ptr[i]; k < ptr[i + 1]; k++) {
-            const int j = col[k];
-            if (j <= i) continue;
-            int q = fill[j]++;
-            fc[q] = i;
-            fv[q] = val[k];
-        }
-    for (int i = 0; i < n; i++)
-        for (int k = ptr[i]; k < ptr[i + 1]; k++) {
-            const int j = col[k];
-            if (j < i) continue;
-            int q = fill[i]++;
-            fc[q] = j;
-            fv[q] = val[k];
-        }
 }
 
 // maxmsteps per package (AFIIM/src/bicgstab.cpp:114, PDFIIM/src/bicgstab.cpp:106,
@@ -1564,7 +1338,7 @@ static int make_bins(RowBins &b, const DevCsr<V> &m, hipStream_t s, int short_ma
     CM_HIP(hipMemcpyAsync(ptr.data(), m.ptr, (size_t)(m.n + 1) * sizeof(int), hipMemcpyDeviceToHost, s));
     CM_HIP(hipStreamSynchronize(s));
     std::vector<int> sh, lo;
-    for (int i = 0; i < m.n; i++) (ptr[i + 1] - ptr[i] > short_max ? lo : sh).push_back(i);
+    row_bins(m.n, ptr.data(), short_max, sh, lo);
     (void)hipFree(b.shortr);
     (void)hipFree(b.longr);
     b.shortr = b.longr = nullptr;

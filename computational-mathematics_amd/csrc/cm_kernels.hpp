@@ -34,12 +34,10 @@
 
 #include <cstdint>
 
+#include "cm_layout.hpp"
+
 namespace cm {
 
-constexpr int BS = 256;       // threads per workgroup (4 waves of 64)
-constexpr int TILE = 2048;    // default nonzeros staged per SpMV workgroup (tunable: 1024, 2048, 4096)
-constexpr int ROWCAP = 1024;  // rows per SpMV workgroup
-constexpr int WAVE = 64;
 constexpr int MAX_RED_BLOCKS = 2048;  // workgroups of the reducing vector kernels
 
 // Solve state kept in device memory.  variant: 0 AFIIM, 1 PDFIIM, 2 ACIFIM.
@@ -633,8 +631,6 @@ __global__ __launch_bounds__(BS) void spmv_vl16(const int *__restrict__ ptr, con
 // in its own nonzero order by one thread (no LDS staging, no shuffles), so
 // the FP64 row sums equal a sequential row loop's bit for bit.
 // Taken for matrices whose rows are at most SLOT_MAXLEN long.
-constexpr int SLOT_ROWS = BS;
-constexpr int SLOT_MAXLEN = 64;
 
 // hdr: per block {first row, rows, data start, slot-table start, slots};
 // lrow: per row position {row length << 16 | block-relative row};
@@ -703,8 +699,6 @@ __global__ __launch_bounds__(BS) void spmv_slot(const int *__restrict__ hdr, con
 // diagonals are taken CHUNK at a time with all their loads in flight.  Each
 // product is rounded before it is added (no fused multiply-add), in diagonal
 // order, then the remainder in column order.
-constexpr int DIA_MAX = 32;
-constexpr int DIA_ALIGN = 8;
 
 template <class V, int RPT>
 struct DiaVec {  // RPT values of V loaded as one aligned vector

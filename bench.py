@@ -414,7 +414,7 @@ def bench_block(args, w):
         out["rel_error_vs_oracle_column0"] = {"mixed_block": _rel(Xmx[:, 0], x_or), "fp64_single": _rel(X64[:, 0], x_or)}
     else:
         out["cpu_baseline"] = None
-    print(json.dumps(out), flush=True)
+    return out
 
 
 def _progress(msg):
@@ -612,7 +612,7 @@ def bench_serve(args, w):
         "note": "solves/s counts right-hand-side columns over the replay (first arrival to last completion); "
                 "with arrivals slower than service it is arrival-bound and latency is the figure of merit",
     }
-    print(json.dumps(out), flush=True)
+    return out
 
 
 def main():
@@ -646,6 +646,10 @@ def main():
     ap.add_argument("--pmc-summary", nargs=5, metavar=("FETCH_DB", "WRITE_DB", "COPY_FETCH_DB", "COPY_WRITE_DB", "BENCH_JSON"),
                     help="summarize rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE result databases of this workload's "
                          "bench and of --calibrate-copy into profiles/traffic_<workload>.json (no GPU use)")
+    ap.add_argument("--legs", default=None,
+                    help="comma-separated workloads run after the headline one in compact form and reported under "
+                         "'legs' (default with the default workload: " + ",".join(DEFAULT_LEGS) + "; 'none' "
+                         "to skip)")
     ap.add_argument("--trace-summary", metavar="TRACE_DB",
                     help="print the per-kernel table of a rocprofv3 --kernel-trace result database (no GPU use)")
     args = ap.parse_args()
@@ -677,11 +681,79 @@ def main():
     if cmamd.device_count() < 1:
         print("bench.py: no GPU visible", file=sys.stderr)
         sys.exit(1)
-    w = WORKLOADS[args.workload]
-    if args.workload == "st27-100-rhs16":
+    out = run_workload(args, args.workload)
+    legs = args.legs if args.legs is not None else (",".join(DEFAULT_LEGS) if args.workload == "cd3d-128" else "none")
+    if legs != "none":
+        out["legs"] = {}
+        for name in legs.split(","):
+            if name == args.workload:
+                continue
+            _progress(f"leg {name}")
+            t0 = time.perf_counter()
+            leg = run_workload(_leg_args(args, name), name)
+            out["legs"][name] = _leg_summary(leg, time.perf_counter() - t0)
+    print(json.dumps(out), flush=True)
+
+
+# The other configurations, run after the headline workload in compact form
+# (fewer steps and requests, short CPU baselines) so that a default run
+# reports every configuration in a few minutes.
+DEFAULT_LEGS = ("cd3d-128-pe5", "st27-100-rhs16", "serve-mix", "cd2d-100")
+LEG_SETTINGS = {
+    "cd3d-128-pe5": dict(steps=5, warmup=1, cpu_seconds=3.0),
+    "st27-100-rhs16": dict(steps=5, warmup=1, cpu_seconds=3.0),
+    "serve-mix": dict(serve_requests=2000, serve_fp64_requests=300),
+    "cd2d-100": dict(steps=100, warmup=5, cpu_seconds=3.0),
+}
+
+
+def _leg_args(args, name):
+    a = argparse.Namespace(**vars(args))
+    a.workload = name
+    for k, v in LEG_SETTINGS.get(name, {}).items():
+        setattr(a, k, v)
+    return a
+
+
+def _leg_summary(out, wall_s):
+    """The figures of a workload's full line that a compact leg reports."""
+    keep = ("value", "unit", "ms_per_step", "steps", "dtype", "latency_ms", "queue_ms", "service_ms",
+            "iterations", "info", "true_rel_residual", "true_rel_residual_max", "forward_error_vs_xstar",
+            "forward_error_vs_xstar_max", "rel_error_vs_oracle", "rel_error_vs_oracle_column0",
+            "rel_error_vs_fp64_solution_max", "outer_steps", "inner_passes", "fp64_one_at_a_time", "fp64_block",
+            "speedup_vs_fp64_one_at_a_time", "pcie_inclusive", "info_nonzero", "mean_batch_columns")
+    leg = {k: out[k] for k in keep if k in out}
+    leg["workload"] = out["config"]["workload"]
+    if out.get("mixed_precision"):
+        leg["mixed_precision"] = {k: {q: v[q] for q in ("value", "outer_steps", "inner_passes", "info",
+                                                           "true_rel_residual", "rel_error_vs_fp64_solution")}
+                                  for k, v in out["mixed_precision"].items()}
+        leg["mixed_speedup_vs_fp64"] = {k: round(v["value"] / out["value"], 3)
+                                        for k, v in out["mixed_precision"].items()}
+    if out.get("roofline"):
+        r = out["roofline"]
+        leg["roofline"] = {k: r[k] for k in ("bound", "kernel", "achieved", "peak", "unit", "frac", "traffic",
+                                              "avg_us", "format_frac") if k in r}
+    if out.get("cpu_baseline"):
+        c = out["cpu_baseline"]
+        leg["cpu_baseline"] = {k: c[k] for k in ("value", "unit", "cores", "kind", "sample") if k in c}
+    leg["wall_s"] = round(wall_s, 1)
+    return leg
+
+
+def run_workload(args, name):
+    w = WORKLOADS[name]
+    if name == "st27-100-rhs16":
         return bench_block(args, w)
-    if args.workload == "serve-mix":
+    if name == "serve-mix":
         return bench_serve(args, w)
+    return bench_single(args, w)
+
+
+def bench_single(args, w):
+    """Single-system workload: one step = one full FP64 solve on the resident
+    system, then the mixed-precision legs on the same system."""
+    import cmamd
     A, xs, b, F, t_build = build_workload(w)
     print(f"[bench] workload {args.workload}: n={A.n} nnz(A)={A.nnz} nnz(G)={F.G.nnz} "
           f"nnz(H)={F.H.nnz} factors built in {t_build:.1f}s", file=sys.stderr, flush=True)
@@ -823,7 +895,7 @@ def main():
         out["rel_error_vs_oracle"] = {"fp64": _rel(x, x_or), **{k: _rel(v, x_or) for k, v in xs_mixed.items()}}
     else:
         out["cpu_baseline"] = None
-    print(json.dumps(out), flush=True)
+    return out
 
 
 if __name__ == "__main__":

@@ -33,6 +33,7 @@
 #include <cstring>
 #include <iostream>
 #include <memory>
+#include <mutex>
 #include <string>
 #include <type_traits>
 #include <vector>
@@ -2811,6 +2812,119 @@ int cm_batch_solve(int nitems, const cm_system *systems, const double *const *B,
     }
     return runner->run(sys, nitems, sys_of.data(), B, X, tol, NMAX, restart_true, resval, iter, info, nullptr,
                        stream);
+}
+
+// ---------------------------------------- batched solves without allocation
+//
+// cm_batch_solve_noalloc runs k_batch_solve with every buffer beyond the
+// caller's inputs and outputs in one file-scope device array: the queue
+// head, the system and item tables and the per-workgroup work arena.  Host
+// code never passes the array's name to a kernel or a copy (in host code it
+// names a host-side stand-in): the device address of each GPU's copy comes
+// from hipGetSymbolAddress, taken once per device with that device current.
+// Host-side tables are file-scope arrays as well, so a call allocates
+// nothing, on the device or the host.  Calls are serialized by a lock; the
+// array keeps the last call's contents (the queue head is cleared by a
+// kernel at the start of each call).
+namespace {
+constexpr int NA_MAX_ITEMS = 4096, NA_MAX_SYS = 256;
+constexpr size_t NA_ALIGN = 256;
+constexpr size_t na_round(size_t b) { return (b + NA_ALIGN - 1) / NA_ALIGN * NA_ALIGN; }
+constexpr size_t NA_HEAD = 0;
+constexpr size_t NA_SYS = NA_ALIGN;
+constexpr size_t NA_ITEMS = NA_SYS + na_round(NA_MAX_SYS * sizeof(BatchSys));
+constexpr size_t NA_ARENA = NA_ITEMS + na_round(NA_MAX_ITEMS * sizeof(BatchItem));
+constexpr size_t NA_ARENA_BYTES = (size_t)64 << 20;  // 64 MiB of work vectors
+constexpr size_t NA_SCRATCH_BYTES = NA_ARENA + NA_ARENA_BYTES;
+constexpr int NA_MAX_DEVICES = 64;
+}  // namespace
+
+static __device__ __align__(256) unsigned char scratch[NA_SCRATCH_BYTES];
+
+static std::mutex g_na_mu;
+static unsigned char *g_na_scratch[NA_MAX_DEVICES];  // device address of `scratch` per device
+static BatchSys g_na_sys[NA_MAX_SYS];
+static BatchItem g_na_items[NA_MAX_ITEMS];
+static const System *g_na_sysptr[NA_MAX_SYS];
+static int g_na_sys_of[NA_MAX_ITEMS], g_na_order[NA_MAX_ITEMS];
+
+int cm_batch_solve_noalloc(int nitems, const cm_system *systems, const double *const *dB, double *const *dX,
+                           const double *tol, const int *NMAX, int restart_true, double *resval, int *iter,
+                           int *info, void *stream)
+{
+    if (nitems < 0 || (nitems > 0 && (!systems || !dB || !dX || !tol || !NMAX)))
+        return fail(CM_ERR_ARG, "null argument");
+    if (nitems == 0) return CM_OK;
+    if (nitems > NA_MAX_ITEMS)
+        return fail(CM_ERR_ARG, "cm_batch_solve_noalloc: more than " + std::to_string(NA_MAX_ITEMS) + " items");
+    std::lock_guard<std::mutex> lock(g_na_mu);
+    int dev = 0, cus = 256;
+    CM_HIP(hipGetDevice(&dev));
+    if (dev < 0 || dev >= NA_MAX_DEVICES) return fail(CM_ERR_ARG, "device index out of range");
+    if (!g_na_scratch[dev]) {
+        void *q = nullptr;
+        CM_HIP(hipGetSymbolAddress(&q, HIP_SYMBOL(scratch)));
+        g_na_scratch[dev] = static_cast<unsigned char *>(q);
+    }
+    unsigned char *base = g_na_scratch[dev];
+    CM_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    // distinct systems in first-appearance order
+    int ns = 0, maxn = 0;
+    for (int i = 0; i < nitems; i++) {
+        if (!systems[i] || !dB[i] || !dX[i]) return fail(CM_ERR_ARG, "null system or vector");
+        const System *p = &systems[i]->sys;
+        int q = 0;
+        while (q < ns && g_na_sysptr[q] != p) q++;
+        if (q == ns) {
+            if (ns == NA_MAX_SYS)
+                return fail(CM_ERR_ARG, "cm_batch_solve_noalloc: more than " + std::to_string(NA_MAX_SYS) +
+                                            " distinct systems");
+            const K64 &k = *p->k64;
+            if (k.variant == 2 && k.mode != 'E') return fail(CM_ERR_UNSUPPORTED, "batched solves use apply mode E");
+            const DevCsr<double> &H = k.sym ? k.GT : k.H;
+            g_na_sys[ns] = BatchSys{k.n, k.variant, max_div_steps(k.variant, k.n), 0, k.A.ptr, k.A.col, k.A.val,
+                                    k.G.ptr, k.G.col, k.G.val, H.ptr, H.col, H.val, k.idv};
+            g_na_sysptr[ns++] = p;
+            maxn = std::max(maxn, k.n);
+        }
+        g_na_sys_of[i] = q;
+    }
+    const size_t stride = ((size_t)9 * maxn + 63) / 64 * 64;  // doubles per workgroup
+    const size_t slots = NA_ARENA_BYTES / (stride * sizeof(double));
+    if (slots < 1)
+        return fail(CM_ERR_ARG, "cm_batch_solve_noalloc: a system of " + std::to_string(maxn) +
+                                    " rows needs more than the " + std::to_string(NA_ARENA_BYTES >> 20) +
+                                    " MiB work arena");
+    const int grid = (int)std::min<size_t>({(size_t)nitems, (size_t)cus, slots});
+    // largest systems first; ties in item order
+    for (int i = 0; i < nitems; i++) g_na_order[i] = i;
+    std::sort(g_na_order, g_na_order + nitems, [](int a, int b) {
+        const int na = g_na_sysptr[g_na_sys_of[a]]->n, nb = g_na_sysptr[g_na_sys_of[b]]->n;
+        return na != nb ? na > nb : a < b;
+    });
+    for (int q = 0; q < nitems; q++) {
+        const int i = g_na_order[q];
+        g_na_items[q] = BatchItem{g_na_sys_of[i], NMAX[i], tol[i], dB[i], dX[i], 0.0, 0, 0, 0, 0};
+    }
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    int *head = reinterpret_cast<int *>(base + NA_HEAD);
+    BatchSys *dsys = reinterpret_cast<BatchSys *>(base + NA_SYS);
+    BatchItem *ditems = reinterpret_cast<BatchItem *>(base + NA_ITEMS);
+    double *arena = reinterpret_cast<double *>(base + NA_ARENA);
+    CM_HIP(hipMemcpyAsync(dsys, g_na_sys, ns * sizeof(BatchSys), hipMemcpyHostToDevice, s));
+    CM_HIP(hipMemcpyAsync(ditems, g_na_items, (size_t)nitems * sizeof(BatchItem), hipMemcpyHostToDevice, s));
+    k_zero_int<<<1, 1, 0, s>>>(head);
+    k_batch_solve<<<grid, PBS, 0, s>>>(nitems, dsys, ditems, arena, stride, head, restart_true);
+    CM_HIP(hipGetLastError());
+    CM_HIP(hipMemcpyAsync(g_na_items, ditems, (size_t)nitems * sizeof(BatchItem), hipMemcpyDeviceToHost, s));
+    CM_HIP(hipStreamSynchronize(s));
+    for (int q = 0; q < nitems; q++) {
+        const int i = g_na_order[q];
+        if (resval) resval[i] = g_na_items[q].resval;
+        if (iter) iter[i] = g_na_items[q].iter;
+        if (info && g_na_items[q].info >= 0) info[i] = g_na_items[q].info;  // untouched when NMAX < 0
+    }
+    return CM_OK;
 }
 
 int cm_system_block_release(cm_system h)

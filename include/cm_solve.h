@@ -146,6 +146,25 @@ int cm_system_block_graphs(cm_system sys, int on);
 int cm_batch_solve(int nitems, const cm_system *systems, const double *const *B, double *const *X,
                    const double *tol, const int *NMAX, int restart_true, double *resval, int *iter, int *info);
 
+/* cm_batch_solve without any allocation, for callers that cannot allocate
+ * device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  No counterpart in the reference.
+ * Same items, per-item results and loop as cm_batch_solve, but B[i] and X[i]
+ * are DEVICE pointers (n doubles of item i's system; X[i] holds the initial
+ * guess and receives the solution in place), and the work runs on `stream`
+ * (a hipStream_t; NULL = the default stream; the call returns when it is
+ * done).  The queue, the item and system tables and the work vectors live in
+ * one fixed device array of the library (64 MiB of work vectors plus tables,
+ * held on each GPU it runs on until the process ends): at most 4096 items
+ * and 256 distinct systems per call, and 9 n_max doubles per workgroup must
+ * fit the 64 MiB (n_max up to about 930,000 rows; fewer rows run more items
+ * at once); larger calls return CM_ERR_ARG.  Calls are serialized inside
+ * the library; concurrent calls from several processes on one GPU use
+ * separate copies. */
+int cm_batch_solve_noalloc(int nitems, const cm_system *systems, const double *const *B, double *const *X,
+                           const double *tol, const int *NMAX, int restart_true, double *resval, int *iter,
+                           int *info, void *stream);
+
 /* Kernel timing of the current block's SpMM y = M X, M = 'A', 'G' or 'H'
  * (first factor; G^T on the symmetric PDFIIM branch), in FP64 (bits 64) or
  * the inner precision (32, 16; cm_system_enable_mixed first): reps

@@ -208,6 +208,21 @@ def pdfiim(A: Csr, prefill, lfill, postfill, ftol, warpsize=-1, filttype="I", pa
                    _take_csr(lib, A.n, Hv, Hj, Hi))
 
 
+def pdfiim_device(A: Csr, prefill, lfill, postfill, ftol, warpsize=-1, filttype="I", patt="B"):
+    """PDFIIM factors built on the GPU (libcmsolve.so); the host producer's
+    factors bit for bit.  Returns (Factors, construction kernels' device ms)."""
+    lib = solve_lib()
+    hl = host_lib()  # releases the returned new[] arrays
+    ptr, idx, val = _csr_c(A)
+    outs = [C.c_void_p() for _ in range(7)]
+    ms = C.c_double()
+    _check(lib.cm_pdfiim_device(A.n, _dp(val), _ip(idx), _ip(ptr), *[C.byref(o) for o in outs], prefill, lfill,
+                                postfill, ftol, warpsize, filttype.encode(), patt.encode(), C.byref(ms)))
+    Gv, Gj, Gi, IDv, Hv, Hj, Hi = outs
+    return (Factors(_take_csr(hl, A.n, Gv, Gj, Gi), _take(hl, IDv.value, A.n, np.float64),
+                    _take_csr(hl, A.n, Hv, Hj, Hi)), ms.value)
+
+
 def read_mtx(path, symmetric_mode=0):
     lib = host_lib()
     n, nnz, sym = C.c_int(), C.c_int(), C.c_int()
@@ -286,7 +301,7 @@ SOLVE_SYMBOLS = [
     "cm_system_block_time_spmm", "cm_batch_solve", "cm_system_autotune", "cm_system_block_autotune",
     "cm_system_spmv_format", "cm_system_spmv_formats", "cm_system_profile_sampled", "cm_system_block_dia", "cm_last_error", "cm_device_count",
     "cm_server_create", "cm_server_configure", "cm_server_destroy", "cm_server_add_system", "cm_server_submit", "cm_server_wait",
-    "cm_server_stats",
+    "cm_server_stats", "cm_batch_solve_noalloc", "cm_pdfiim_device",
 ]
 
 
@@ -333,6 +348,9 @@ def _type_solve(lib):
     lib.cm_batch_solve.argtypes = [C.c_int, C.POINTER(vp), C.POINTER(PD), C.POINTER(PD), PD, PI, C.c_int, PD,
                                    PI, PI]
     lib.cm_batch_solve.restype = C.c_int
+    lib.cm_pdfiim_device.argtypes = [C.c_int, PD, PI, PI] + [C.POINTER(vp)] * 7 + [
+        C.c_int, C.c_int, C.c_int, C.c_double, C.c_int, C.c_char, C.c_char, PD]
+    lib.cm_pdfiim_device.restype = C.c_int
     lib.cm_system_autotune.argtypes = [vp, C.c_int, PI]
     lib.cm_system_autotune.restype = C.c_int
     lib.cm_system_spmv_format.argtypes = [vp, C.c_int, C.c_int, C.c_int]

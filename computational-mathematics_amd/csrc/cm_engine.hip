@@ -43,6 +43,7 @@
 #include "cm_batch.hpp"
 #include "cm_block.hpp"
 #include "cm_trsv.hpp"
+#include "cm_pdfiim.hpp"
 
 namespace cm {
 
@@ -2470,6 +2471,43 @@ struct BatchRunner {
     size_t hitems_cap = 0;
 };
 
+// Helpers of the GPU PDFIIM construction (cm_pdfiim_device).
+namespace {
+template <class T>
+T *pd_new_copy(const std::vector<T> &v)
+{
+    T *p = new T[std::max<size_t>(v.size(), 1)];
+    std::copy(v.begin(), v.end(), p);
+    return p;
+}
+
+// Device buffers of one construction, freed on every exit.
+struct PdBuffers {
+    std::vector<void *> held;
+    hipStream_t stream = nullptr;
+    template <class T>
+    int alloc(T **d, size_t count)
+    {
+        CM_TRY(dev_alloc(d, count));
+        held.push_back((void *)*d);
+        return CM_OK;
+    }
+    template <class T>
+    int upload(T **d, const T *h, size_t count)
+    {
+        CM_TRY(alloc(d, count));
+        if (count) CM_HIP(hipMemcpyAsync(*d, h, count * sizeof(T), hipMemcpyHostToDevice, stream));
+        return CM_OK;
+    }
+    ~PdBuffers()
+    {
+        if (stream) (void)hipStreamSynchronize(stream);
+        for (void *q : held) (void)hipFree(q);
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+};
+}  // namespace
+
 struct cm_system_s {
     System sys;
 };
@@ -2924,6 +2962,220 @@ int cm_batch_solve_noalloc(int nitems, const cm_system *systems, const double *c
         if (iter) iter[i] = g_na_items[q].iter;
         if (info && g_na_items[q].info >= 0) info[i] = g_na_items[q].info;  // untouched when NMAX < 0
     }
+    return CM_OK;
+}
+
+// ------------------------------------------- PDFIIM construction on the GPU
+
+int cm_pdfiim_device(int n, const double *Av, const int *Aj, const int *Ai, double **Gval, int **Gindj,
+                     int **Gindi, double **IDv, double **Hval, int **Hindj, int **Hindi, int prefill, int lfill,
+                     int postfill, double ftol, int warpsize, char filttype, char patt, double *device_ms)
+{
+    if (!Gval || !Gindj || !Gindi || !IDv || !Hval || !Hindj || !Hindi) return fail(CM_ERR_ARG, "null output");
+    *Gval = *Hval = *IDv = nullptr;
+    *Gindj = *Gindi = *Hindj = *Hindi = nullptr;
+    if (n <= 0 || !Av || !Aj || !Ai) return fail(CM_ERR_ARG, "null or empty matrix");
+    if (patt != 'B' && patt != 'L' && patt != 'U') return fail(CM_ERR_ARG, "patt must be 'B', 'L' or 'U'");
+    // the reference's 'L'/'U' level-set expansion walks a stale list cursor
+    // past the first pass (PDFIIM/src/parPrec.cpp:393-456): undefined there
+    if (patt != 'B' && lfill > 1) return fail(CM_ERR_UNSUPPORTED, "patt 'L'/'U' need lfill == 1");
+    // delete_last below an empty set is undefined in the reference
+    if (warpsize != -1 && warpsize < 1) return fail(CM_ERR_ARG, "warpsize must be -1 or positive");
+    CM_TRY(check_csr(n, Ai, Aj));
+    std::vector<int> tp, tc;
+    std::vector<double> tv;
+    host_transpose(n, Ai, Aj, Av, tp, tc, tv);
+    PdBuffers M;
+    CM_HIP(hipStreamCreateWithFlags(&M.stream, hipStreamNonBlocking));
+    const hipStream_t s = M.stream;
+    const long long nnz = Ai[n];
+    PdCsr A{}, At{};
+    {
+        int *p, *c, *tpd, *tcd;
+        double *v, *tvd;
+        CM_TRY(M.upload(&p, Ai, (size_t)n + 1));
+        CM_TRY(M.upload(&c, Aj, (size_t)nnz));
+        CM_TRY(M.upload(&v, Av, (size_t)nnz));
+        CM_TRY(M.upload(&tpd, tp.data(), tp.size()));
+        CM_TRY(M.upload(&tcd, tc.data(), tc.size()));
+        CM_TRY(M.upload(&tvd, tv.data(), tv.size()));
+        A = PdCsr{p, c, v};
+        At = PdCsr{tpd, tcd, tvd};
+    }
+    const PdParams P{n, prefill, lfill, postfill, warpsize, ftol, std::max(1e-4, ftol), filttype, patt};
+    int *gcnt, *hcnt, *flag;
+    double *dinv;
+    CM_TRY(M.alloc(&gcnt, (size_t)n));
+    CM_TRY(M.alloc(&hcnt, (size_t)n));
+    CM_TRY(M.alloc(&flag, (size_t)n));
+    CM_TRY(M.alloc(&dinv, (size_t)n));
+    hipEvent_t e0, e1;
+    CM_HIP(hipEventCreate(&e0));
+    CM_HIP(hipEventCreate(&e1));
+    float ms_total = 0.f;
+    // tier 1: every row 1..n-1 with patterns of at most 32 indices
+    const int nrows = n - 1;
+    constexpr int OUT1 = 33, OUT2 = 65;
+    int *gi1 = nullptr, *hi1 = nullptr;
+    double *gv1 = nullptr, *hv1 = nullptr;
+    std::vector<int> hflag(n, 0), rows2, rows3;
+    if (nrows > 0) {
+        CM_TRY(M.alloc(&gi1, (size_t)nrows * OUT1));
+        CM_TRY(M.alloc(&hi1, (size_t)nrows * OUT1));
+        CM_TRY(M.alloc(&gv1, (size_t)nrows * OUT1));
+        CM_TRY(M.alloc(&hv1, (size_t)nrows * OUT1));
+        const PdOut O{gcnt, hcnt, flag, gi1, hi1, gv1, hv1, dinv, OUT1};
+        const int grid = std::min(nrows, 16384);
+        CM_HIP(hipEventRecord(e0, s));
+        k_pdfiim_rows<32><<<grid, PD_WAVE, 0, s>>>(nrows, nullptr, P, A, At, O);
+        CM_HIP(hipGetLastError());
+        CM_HIP(hipEventRecord(e1, s));
+        CM_HIP(hipMemcpyAsync(hflag.data() + 1, flag + 1, (size_t)nrows * sizeof(int), hipMemcpyDeviceToHost, s));
+        CM_HIP(hipStreamSynchronize(s));
+        float ms = 0.f;
+        CM_HIP(hipEventElapsedTime(&ms, e0, e1));
+        ms_total += ms;
+        for (int i = 1; i < n; i++)
+            if (hflag[i]) rows2.push_back(i);
+    }
+    // tier 2: the rows whose pattern outgrew 32 indices, with room for 64
+    int *gi2 = nullptr, *hi2 = nullptr, *rows2d = nullptr;
+    double *gv2 = nullptr, *hv2 = nullptr;
+    const int n2 = (int)rows2.size();
+    if (n2) {
+        CM_TRY(M.upload(&rows2d, rows2.data(), rows2.size()));
+        CM_TRY(M.alloc(&gi2, (size_t)n2 * OUT2));
+        CM_TRY(M.alloc(&hi2, (size_t)n2 * OUT2));
+        CM_TRY(M.alloc(&gv2, (size_t)n2 * OUT2));
+        CM_TRY(M.alloc(&hv2, (size_t)n2 * OUT2));
+        const PdOut O{gcnt, hcnt, flag, gi2, hi2, gv2, hv2, dinv, OUT2};
+        CM_HIP(hipEventRecord(e0, s));
+        k_pdfiim_rows<64><<<std::min(n2, 16384), PD_WAVE, 0, s>>>(n2, rows2d, P, A, At, O);
+        CM_HIP(hipGetLastError());
+        CM_HIP(hipEventRecord(e1, s));
+        std::vector<int> f2(n);
+        CM_HIP(hipMemcpyAsync(f2.data(), flag, (size_t)n * sizeof(int), hipMemcpyDeviceToHost, s));
+        CM_HIP(hipStreamSynchronize(s));
+        float ms = 0.f;
+        CM_HIP(hipEventElapsedTime(&ms, e0, e1));
+        ms_total += ms;
+        for (int i : rows2)
+            if (f2[i]) rows3.push_back(i);
+    }
+    // tier 3: patterns of up to PD_BIG indices, the dense systems in global memory
+    int *gi3 = nullptr, *hi3 = nullptr, *rows3d = nullptr;
+    double *gv3 = nullptr, *hv3 = nullptr;
+    const int n3 = (int)rows3.size();
+    constexpr int OUT3 = PD_BIG + 1;
+    if (n3) {
+        const int grid3 = std::min(n3, 256);
+        PdBigSlab *slabs;
+        CM_TRY(M.alloc(&slabs, (size_t)grid3));
+        CM_TRY(M.upload(&rows3d, rows3.data(), rows3.size()));
+        CM_TRY(M.alloc(&gi3, (size_t)n3 * OUT3));
+        CM_TRY(M.alloc(&hi3, (size_t)n3 * OUT3));
+        CM_TRY(M.alloc(&gv3, (size_t)n3 * OUT3));
+        CM_TRY(M.alloc(&hv3, (size_t)n3 * OUT3));
+        const PdOut O{gcnt, hcnt, flag, gi3, hi3, gv3, hv3, dinv, OUT3};
+        CM_HIP(hipEventRecord(e0, s));
+        k_pdfiim_rows_big<<<grid3, PD_WAVE, 0, s>>>(n3, rows3d, P, A, At, O, slabs);
+        CM_HIP(hipGetLastError());
+        CM_HIP(hipEventRecord(e1, s));
+        std::vector<int> f3(n);
+        CM_HIP(hipMemcpyAsync(f3.data(), flag, (size_t)n * sizeof(int), hipMemcpyDeviceToHost, s));
+        CM_HIP(hipStreamSynchronize(s));
+        float ms = 0.f;
+        CM_HIP(hipEventElapsedTime(&ms, e0, e1));
+        ms_total += ms;
+        for (int i : rows3)
+            if (f3[i]) {
+                (void)hipEventDestroy(e0);
+                (void)hipEventDestroy(e1);
+                return fail(CM_ERR_UNSUPPORTED, "PDFIIM pattern of row " + std::to_string(i) + " exceeds " +
+                                                    std::to_string(PD_BIG) +
+                                                    " indices; build these factors with cm_pdfiim");
+            }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    // counts and D back; packed offsets (row 0: the unit entry alone)
+    std::vector<int> gc(n, 1), hc(n, 1);
+    std::vector<double> dv(n);
+    if (nrows > 0) {
+        CM_HIP(hipMemcpyAsync(gc.data() + 1, gcnt + 1, (size_t)nrows * sizeof(int), hipMemcpyDeviceToHost, s));
+        CM_HIP(hipMemcpyAsync(hc.data() + 1, hcnt + 1, (size_t)nrows * sizeof(int), hipMemcpyDeviceToHost, s));
+        CM_HIP(hipMemcpyAsync(dv.data() + 1, dinv + 1, (size_t)nrows * sizeof(double), hipMemcpyDeviceToHost, s));
+        CM_HIP(hipStreamSynchronize(s));
+    }
+    dv[0] = 1. / Av[0];  // PDFIIM/src/parPrec.cpp:132
+    std::vector<long long> go(n + 1, 0), ho(n + 1, 0);
+    for (int i = 0; i < n; i++) {
+        go[i + 1] = go[i] + gc[i];
+        ho[i + 1] = ho[i] + hc[i];
+    }
+    if (go[n] > INT_MAX || ho[n] > INT_MAX) return fail(CM_ERR_UNSUPPORTED, "factor has more than 2^31 entries");
+    std::vector<int> gpk(go[n]), hpk(ho[n]);
+    std::vector<double> gvk(go[n]), hvk(ho[n]);
+    gpk[0] = 0;
+    gvk[0] = 1.0;
+    hpk[0] = 0;
+    hvk[0] = 1.0;
+    if (nrows > 0) {
+        long long *god, *hod;
+        int *gpd, *hpd;
+        double *gvd, *hvd;
+        CM_TRY(M.upload(&god, go.data(), go.size()));
+        CM_TRY(M.upload(&hod, ho.data(), ho.size()));
+        CM_TRY(M.alloc(&gpd, (size_t)go[n]));
+        CM_TRY(M.alloc(&hpd, (size_t)ho[n]));
+        CM_TRY(M.alloc(&gvd, (size_t)go[n]));
+        CM_TRY(M.alloc(&hvd, (size_t)ho[n]));
+        const int g1 = (int)std::min<long long>((nrows + BS - 1) / BS, 8192);
+        k_pd_pack<<<g1, BS, 0, s>>>(nrows, nullptr, OUT1, gcnt, god, gi1, gv1, gpd, gvd);
+        k_pd_pack<<<g1, BS, 0, s>>>(nrows, nullptr, OUT1, hcnt, hod, hi1, hv1, hpd, hvd);
+        if (n2) {  // the tier-2 rows overwrite their (stale) tier-1 slots' packed places
+            const int g2 = (int)std::min<long long>((n2 + BS - 1) / BS, 8192);
+            k_pd_pack<<<g2, BS, 0, s>>>(n2, rows2d, OUT2, gcnt, god, gi2, gv2, gpd, gvd);
+            k_pd_pack<<<g2, BS, 0, s>>>(n2, rows2d, OUT2, hcnt, hod, hi2, hv2, hpd, hvd);
+        }
+        if (n3) {  // and the tier-3 rows theirs
+            const int g3 = (int)std::min<long long>((n3 + BS - 1) / BS, 8192);
+            k_pd_pack<<<g3, BS, 0, s>>>(n3, rows3d, OUT3, gcnt, god, gi3, gv3, gpd, gvd);
+            k_pd_pack<<<g3, BS, 0, s>>>(n3, rows3d, OUT3, hcnt, hod, hi3, hv3, hpd, hvd);
+        }
+        CM_HIP(hipGetLastError());
+        const size_t g0 = (size_t)go[1], h0 = (size_t)ho[1];
+        CM_HIP(hipMemcpyAsync(gpk.data() + g0, gpd + g0, (go[n] - g0) * sizeof(int), hipMemcpyDeviceToHost, s));
+        CM_HIP(hipMemcpyAsync(gvk.data() + g0, gvd + g0, (go[n] - g0) * sizeof(double), hipMemcpyDeviceToHost, s));
+        CM_HIP(hipMemcpyAsync(hpk.data() + h0, hpd + h0, (ho[n] - h0) * sizeof(int), hipMemcpyDeviceToHost, s));
+        CM_HIP(hipMemcpyAsync(hvk.data() + h0, hvd + h0, (ho[n] - h0) * sizeof(double), hipMemcpyDeviceToHost, s));
+        CM_HIP(hipStreamSynchronize(s));
+    }
+    // H rows are the packed row lists; G by rows from its column lists
+    // (columns ascending in every row, the unit diagonal first)
+    std::vector<int> hptr(n + 1), gptr(n + 1, 0);
+    for (int i = 0; i <= n; i++) hptr[i] = (int)ho[i];
+    for (long long k = 0; k < go[n]; k++) gptr[gpk[k] + 1]++;
+    for (int i = 0; i < n; i++) gptr[i + 1] += gptr[i];
+    std::vector<int> gj(go[n]);
+    std::vector<double> gvv(go[n]);
+    {
+        std::vector<int> fill(gptr.begin(), gptr.end() - 1);
+        for (int i = 0; i < n; i++)
+            for (long long k = go[i]; k < go[i + 1]; k++) {
+                const int q = fill[gpk[k]]++;
+                gj[q] = i;
+                gvv[q] = gvk[k];
+            }
+    }
+    *Gindi = pd_new_copy(gptr);
+    *Gindj = pd_new_copy(gj);
+    *Gval = pd_new_copy(gvv);
+    *Hindi = pd_new_copy(hptr);
+    *Hindj = pd_new_copy(hpk);
+    *Hval = pd_new_copy(hvk);
+    *IDv = pd_new_copy(dv);
+    if (device_ms) *device_ms = ms_total;
     return CM_OK;
 }
 

@@ -114,40 +114,33 @@ void lu_factor(int len, double *B, int *piv)
     }
 }
 
-// Solve B x = b given the LU factors (b overwritten by x).
+// Solve B x = b given the LU factors (b overwritten by x).  Both
+// substitutions are column-oriented: once an unknown is final it is
+// subtracted from every later (forward) or earlier (backward) row, so each
+// entry's operations run in a fixed order that a wave-parallel
+// implementation (one lane per row) reproduces exactly.
 void lu_solve_n(int len, const double *B, const int *piv, double *b)
 {
     for (int j = 0; j < len; j++)
         if (piv[j] != j) std::swap(b[j], b[piv[j]]);
-    for (int r = 0; r < len; r++) {
-        double s = b[r];
-        const double *row = B + (size_t)r * len;
-        for (int c = 0; c < r; c++) s -= row[c] * b[c];
-        b[r] = s;
-    }
-    for (int r = len - 1; r >= 0; r--) {
-        double s = b[r];
-        const double *row = B + (size_t)r * len;
-        for (int c = r + 1; c < len; c++) s -= row[c] * b[c];
-        b[r] = s / row[r];
+    for (int c = 0; c < len; c++)  // L y = b, unit diagonal
+        for (int r = c + 1; r < len; r++) b[r] -= B[(size_t)r * len + c] * b[c];
+    for (int c = len - 1; c >= 0; c--) {  // U x = y
+        b[c] = b[c] / B[(size_t)c * len + c];
+        for (int r = 0; r < c; r++) b[r] -= B[(size_t)r * len + c] * b[c];
     }
 }
 
-// Solve B^T x = b given the LU factors of B (b overwritten by x).
+// Solve B^T x = b given the LU factors of B (b overwritten by x), in the
+// same column-oriented form.
 void lu_solve_t(int len, const double *B, const int *piv, double *b)
 {
-    // U^T y = b
-    for (int r = 0; r < len; r++) {
-        double s = b[r];
-        for (int c = 0; c < r; c++) s -= B[(size_t)c * len + r] * b[c];
-        b[r] = s / B[(size_t)r * len + r];
+    for (int c = 0; c < len; c++) {  // U^T y = b
+        b[c] = b[c] / B[(size_t)c * len + c];
+        for (int r = c + 1; r < len; r++) b[r] -= B[(size_t)c * len + r] * b[c];
     }
-    // L^T z = y (unit diagonal)
-    for (int r = len - 1; r >= 0; r--) {
-        double s = b[r];
-        for (int c = r + 1; c < len; c++) s -= B[(size_t)c * len + r] * b[c];
-        b[r] = s;
-    }
+    for (int c = len - 1; c >= 0; c--)  // L^T z = y, unit diagonal
+        for (int r = 0; r < c; r++) b[r] -= B[(size_t)c * len + r] * b[c];
     for (int j = len - 1; j >= 0; j--)
         if (piv[j] != j) std::swap(b[j], b[piv[j]]);
 }
@@ -184,6 +177,8 @@ extern "C" int cm_pdfiim(int n, const double *Av, const int *Aj, const int *Ai,
     // undefined once a second pass uses the expanded set.  Only the single-pass
     // form, whose expansion never reaches the output, is defined.
     if (patt != 'B' && lfill > 1) return -1;
+    // below 1 (other than -1) the reference's truncation pops an empty set
+    if (warpsize != -1 && warpsize < 1) return -1;
 
     // A^T in CSR (PDFIIM/src/sparseLinearAlgebra.cpp:50-109).
     std::vector<int> tptr(n + 1, 0), tidx(Ai[n]);

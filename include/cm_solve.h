@@ -51,6 +51,22 @@ int cm_acifim_pbicgstab(int n, double *Av, int *Aj, int *Ai, double *b, double t
                         double *Uv, int *Uj, int *Ui, int nr, char mode, double *x,
                         double *resval, int *iter, int verbose, int *info);
 
+/* PDFIIM factors built on the GPU: same parameters, outputs (new[] arrays,
+ * release with cm_free_d / cm_free_i or delete[]) and meaning as cm_pdfiim
+ * in cm_host.h, i.e. the reference's pdfiim() (PDFIIM/src/parPrec.hpp:86-96,
+ * PDFIIM/src/parPrec.cpp:95-588), and the same factors bit for bit as the
+ * host producer.  One wavefront builds one row (pattern growth, the dense
+ * LU solves and the filters), with the dense system in LDS for patterns of
+ * up to 64 indices and in global memory up to 256 (CM_ERR_UNSUPPORTED
+ * beyond, naming the row).  patt 'L'/'U' need
+ * lfill == 1 and warpsize must be -1 or positive, as in cm_pdfiim.
+ * device_ms (optional) receives the construction kernels' device time. */
+int cm_pdfiim_device(int n, const double *Av, const int *Aj, const int *Ai,
+                     double **Gval, int **Gindj, int **Gindi, double **IDv,
+                     double **Hval, int **Hindj, int **Hindi, int prefill, int lfill,
+                     int postfill, double ftol, int warpsize, char filttype, char patt,
+                     double *device_ms);
+
 /* ---- resident systems (A and factors kept in HBM across solves) ---- */
 
 typedef struct cm_system_s *cm_system;

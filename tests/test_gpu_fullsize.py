@@ -1,7 +1,8 @@
 """Parity at the full sizes of the benchmark configurations (the C1 case, the
 2D 100^2 system, runs at its size in test_gpu_parity.py):
 
-* cd3d-128 -- 3D 7-point 128^3 (n = 2,097,152), Peclet 1, PDFIIM(2,2,1,0.05),
+* cd3d-128 -- 3D 7-point 128^3 (n = 2,097,152), Peclet 1, PDFIIM(2,2,1,0.05)
+  (built on the GPU too, equal to the host producer's factors bit for bit),
   FP64 to 1e-8: per-pass residuals within 1e-8 relative over the first 20
   passes (rounding floor 1e-12 ||r_0||), iterations within 2%, same info,
   solution within 1e-6 of the oracle's, true relative residual <= 1e-8;
@@ -88,6 +89,19 @@ def _single_workload(name, variant):
 @pytest.mark.timeout(600)
 def test_fullsize_cd3d_128_pdfiim():
     _single_workload("cd3d-128", "pdfiim")
+
+
+@pytest.mark.timeout(600)
+def test_fullsize_cd3d_128_pdfiim_built_on_gpu():
+    # the 128^3 PDFIIM(2,2,1,0.05) factors built on the GPU equal the host
+    # producer's bit for bit
+    w = workloads.WORKLOADS["cd3d-128"]
+    A, xs, b, F, t_build = workloads.build(w)
+    Fd, ms = cmamd.pdfiim_device(A, *w["args"], -1, "I", "B")
+    print(f"full size cd3d-128: PDFIIM host {t_build:.2f}s, GPU kernels {ms:.1f} ms", flush=True)
+    for a, c in ((Fd.G, F.G), (Fd.H, F.H)):
+        assert np.array_equal(a.ptr, c.ptr) and np.array_equal(a.idx, c.idx) and np.array_equal(a.val, c.val)
+    assert np.array_equal(Fd.idv, F.idv)
 
 
 @pytest.mark.timeout(600)

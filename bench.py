@@ -229,7 +229,8 @@ def cpu_baseline(w, A, b, F, seconds):
     t_start = time.perf_counter()
     while True:
         t0 = time.perf_counter()
-        x, _, it, info, _ = oracle.bicgstab(w["prec"], A, b, F, tol=w["tol"], nmax=w["nmax"])
+        x, _, it, info, _ = oracle.bicgstab(w["prec"], A, b, F, tol=w["tol"], nmax=w["nmax"],
+                                            issym=w.get("sym", False))
         times.append(time.perf_counter() - t0)
         its.append(it)
         if x_or is None:
@@ -698,9 +699,10 @@ def main():
 # The other configurations, run after the headline workload in compact form
 # (fewer steps and requests, short CPU baselines) so that a default run
 # reports every configuration in a few minutes.
-DEFAULT_LEGS = ("cd3d-128-pe5", "st27-100-rhs16", "serve-mix", "cd2d-100")
+DEFAULT_LEGS = ("cd3d-128-pe5", "st27-100-rhs16", "serve-mix", "cd2d-100", "cd3d-128-sym")
 LEG_SETTINGS = {
     "cd3d-128-pe5": dict(steps=5, warmup=1, cpu_seconds=3.0),
+    "cd3d-128-sym": dict(steps=5, warmup=1, cpu_seconds=3.0),
     "st27-100-rhs16": dict(steps=5, warmup=1, cpu_seconds=3.0),
     "serve-mix": dict(serve_requests=2000, serve_fp64_requests=300),
     "cd2d-100": dict(steps=100, warmup=5, cpu_seconds=3.0),
@@ -755,10 +757,17 @@ def bench_single(args, w):
     system, then the mixed-precision legs on the same system."""
     import cmamd
     A, xs, b, F, t_build = build_workload(w)
-    print(f"[bench] workload {args.workload}: n={A.n} nnz(A)={A.nnz} nnz(G)={F.G.nnz} "
-          f"nnz(H)={F.H.nnz} factors built in {t_build:.1f}s", file=sys.stderr, flush=True)
+    sym = bool(w.get("sym", False))  # A is the stored upper triangle; residuals use the full matrix
+    A_full = A
+    if sym:
+        import scipy.sparse as sps
+        U = A.to_scipy()
+        A_full = cmamd.Csr.from_scipy(U + U.T - sps.diags(U.diagonal()))
+    nnz_h = F.H.nnz if F.H is not None else 0
+    print(f"[bench] workload {args.workload}: n={A.n} nnz(A)={A_full.nnz} nnz(G)={F.G.nnz} "
+          f"nnz(H)={nnz_h} factors built in {t_build:.1f}s", file=sys.stderr, flush=True)
 
-    sys_ = cmamd.System(w["prec"], A, F)
+    sys_ = cmamd.System(w["prec"], A, F, a_symmetric=sym)
     if args.no_graphs:
         sys_.graphs(False)
     tuned = None
@@ -804,13 +813,13 @@ def bench_single(args, w):
     dfby = sys_.kernel_format_bytes()[dom]
     sys_.profile(False)
     x = sys_.get_x()
-    true_res = float(np.linalg.norm(b - cmamd.csr_mv_host(A, x)) / np.linalg.norm(b))
+    true_res = float(np.linalg.norm(b - cmamd.csr_mv_host(A_full, x)) / np.linalg.norm(b))
     fwd_err = float(np.linalg.norm(x - xs) / np.linalg.norm(xs))
     # PCIe-inclusive rate: one call of the reference-signature drop-in entry
     # point, which uploads A, the factors and b, solves, and copies x back
     t0 = time.perf_counter()
     if w["prec"] == "pdfiim":
-        dr = cmamd.pdfiim_epbicgstab(A, b, F, tol=w["tol"], nmax=w["nmax"])
+        dr = cmamd.pdfiim_epbicgstab(A, b, F, tol=w["tol"], nmax=w["nmax"], a_symmetric=sym)
     elif w["prec"] == "afiim":
         dr = cmamd.afiim_epbicgstab(A, b, F, tol=w["tol"], nmax=w["nmax"])
     else:
@@ -820,7 +829,7 @@ def bench_single(args, w):
     xs_mixed = {}
     if not args.no_mixed:
         for bits in (32, 16):
-            mixed[f"fp{bits}_factors"] = mixed_leg(sys_, bits, w, A, b, x, args)
+            mixed[f"fp{bits}_factors"] = mixed_leg(sys_, bits, w, A_full, b, x, args)
             xs_mixed[f"fp{bits}_factors"] = sys_.get_x()
     formats = sys_.spmv_formats() if tuned else {}
     sys_.close()
@@ -873,8 +882,8 @@ def bench_single(args, w):
         "metric": METRIC, "value": round(value, 3), "unit": "solves/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": round(1e3 * elapsed / args.steps, 3), "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
-        "config": {"workload": w["desc"], "n": A.n, "nnz_A": A.nnz, "nnz_G": F.G.nnz, "nnz_H": F.H.nnz,
-                   "tol": w["tol"], "nmax": w["nmax"]},
+        "config": {"workload": w["desc"], "n": A.n, "nnz_A": A_full.nnz, "nnz_G": F.G.nnz, "nnz_H": nnz_h,
+                   "tol": w["tol"], "nmax": w["nmax"], "symmetric": sym},
         "latency_ms": {"p50": round(1e3 * float(np.percentile(lat, 50)), 3),
                        "p99": round(1e3 * float(np.percentile(lat, 99)), 3)},
         "iterations": int(np.median(iters)), "info": info, "resval": rv, "true_rel_residual": true_res,

@@ -14,7 +14,7 @@ import time
 
 import numpy as np
 
-from . import acifim, afiim, csr_mv_host, gen_convdiff, gen_stencil27, gen_uniform, pdfiim
+from . import Csr, Factors, acifim, afiim, csr_mv_host, gen_convdiff, gen_stencil27, gen_uniform, pdfiim
 
 WORKLOADS = {
     "cd3d-128": dict(
@@ -39,23 +39,47 @@ WORKLOADS = {
              "per request; FP32-inner / FP64-refinement to true rel-res 1e-8 with continuous batching, vs the "
              "plain FP64 one-at-a-time GPU path on the same requests",
         prec="afiim", tol=1e-8, nmax=1000),
+    "cd3d-128-sym": dict(
+        desc="3D 7-point diffusion 128^3 (n=2097152), log-normal diffusion, no convection (symmetric): A "
+             "held as its upper triangle with mtype SYMMETRIC, PDFIIM(2,2,1,0.05) built on that stored "
+             "triangle as the reference driver does, solve on the symmetric branch (full A on the device, "
+             "M = G D^-1 G^T through an explicit G^T copy); b=A x*, x0=0; plain FP64 BiCGSTAB to rel-res 1e-8",
+        dim=3, N=128, pe=0.0, seed=6, xseed=7, prec="pdfiim", sym=True, args=(2, 2, 1, 0.05), tol=1e-8,
+        nmax=1200),
     "cd2d-100": dict(
         desc="2D 5-point convection-diffusion 100^2 (n=10000, nnz=49600), cell Peclet 0.5; AFIIM eps=0.1; "
              "b=A x*, x0=0; plain FP64 BiCGSTAB to rel-res 1e-8",
         dim=2, N=100, pe=0.5, seed=1, xseed=2, prec="afiim", args=(0.1,), tol=1e-8, nmax=500),
 }
 
+def upper_triangle(A):
+    """The upper triangle (diagonal included) of a CSR matrix: the stored
+    form of a symmetric matrix (mtype SYMMETRIC)."""
+    rows = np.repeat(np.arange(A.n, dtype=np.int32), np.diff(A.ptr))
+    keep = A.idx >= rows
+    ptr = np.zeros(A.n + 1, np.int32)
+    np.cumsum(np.bincount(rows[keep], minlength=A.n), out=ptr[1:])
+    return Csr(A.n, ptr, np.ascontiguousarray(A.idx[keep]), np.ascontiguousarray(A.val[keep]))
+
+
 def build(w):
-    """(A, x*, b, factors, host build seconds) of workload w."""
+    """(A, x*, b, factors, host build seconds) of workload w.  For a
+    symmetric workload A is the stored upper triangle, b = A_full x*, and the
+    factors are built on the stored triangle with H left out (the solve uses
+    G^T)."""
     if w.get("gen") == "stencil27":
         A = gen_stencil27(w["N"], w["frac"], w["lo"], w["hi"], seed=w["seed"])
     else:
         A = gen_convdiff(w["dim"], w["N"], w["pe"], seed=w["seed"])
     xs = gen_uniform(w["xseed"], A.n)
     b = csr_mv_host(A, xs)
+    if w.get("sym"):
+        A = upper_triangle(A)
     t0 = time.perf_counter()
     if w["prec"] == "pdfiim":
         F = pdfiim(A, *w["args"], -1, "I", "B")
+        if w.get("sym"):
+            F = Factors(F.G, F.idv, None)
     elif w["prec"] == "afiim":
         F = afiim(A, *w["args"])
     else:

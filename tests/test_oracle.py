@@ -13,7 +13,7 @@ fix -- never against a retyped copy of its own formulas:
   BiCGSTAB (van der Vorst 1992) written with numpy and dense M;
 * the exit codes against hand-derived cases (NMAX reached; NMAX <= 0; the
   unguarded omega = 0/0 breakdown of an exact preconditioner, where AFIIM and
-  PDFIIM run to NMAX on NaNs and ACIFIM's finite-alpha guard stops at pass 1;
+  PDFIIM run to NMAX on NaNs and ACIFIM's finite-alpha guard stops at pass 2;
   stagnation below rounding; divergence on systems whose singular
   preconditioned operator makes the residual grow on every pass, with the exit
   pass derived from the textbook trace and the reference's counting rule, for
@@ -236,7 +236,7 @@ def test_exit_exact_preconditioner_breakdown():
     A, b, F = _exact_diag()
     x, rv, it, info, _ = oracle.bicgstab("afiim", A, b, F, nmax=7)
     assert (it, info) == (7, 3) and np.isnan(rv)
-    # ACIFIM: pass 1 has alpha = NaN/NaN -> iter = NMAX, info = 3, rc still ||r0||
+    # ACIFIM: pass 2 has alpha = NaN/NaN -> iter = NMAX, info = 3, rc still ||r0||
     x, rv, it, info, _ = oracle.bicgstab("acifim", A, b, F, nmax=7)
     assert (it, info, rv) == (7, 3, 1.0)
 

@@ -24,6 +24,10 @@ constexpr int DIA_MAX = 32;       // diagonals stored densely, at most
 constexpr int DIA_ALIGN = 8;      // diagonal stride: n rounded up to this
 constexpr double DIA_FILL = 0.6;  // a diagonal is stored densely when at least this share of rows has it
 constexpr int SHORT_MAX = 64;     // block products: longer rows go to the wave-per-row bin
+// Diagonal block products: rows taken DIA_RUN_ROWS at a time per lane group,
+// diagonals in runs of at most DIA_RUN_MAX consecutive offsets.
+constexpr int DIA_RUN_ROWS = 4, DIA_RUN_MAX = 4;
+static_assert(DIA_MAX < 256, "a run stores its first diagonal in 8 bits");
 // Column tiles of the long remainders of the diagonal block products: about
 // one tile per LONG_TILE_COLS columns (2 MB of x rows at 16 FP32 columns),
 // at most LONG_TILES_MAX, and at least LONG_SEG_MIN remainder entries per
@@ -257,11 +261,12 @@ inline bool slot_layout(int n, const int *ptr, const int *col, SlotLayout &S)
 // their remainders cut into ntile column tiles of tw columns, stored as a
 // tile-major CSR of (tile, row) segments (lptr, lcol, lsrc; segment t * nlong
 // + q holds long row q's entries with columns in [t tw, (t + 1) tw), in row
-// order).
+// order).  runs: the diagonals as runs of consecutive offsets, at most
+// DIA_RUN_MAX long, each stored as its first diagonal | its length << 8.
 struct DiaLayout {
     int ldn = 0, nd = 0, nlong = 0, ntile = 1, tw = 0;
     long long nseg = 0;
-    std::vector<int> off, dsrc, rptr, rcol, rsrc, lpos, lptr, lcol, lsrc;
+    std::vector<int> off, runs, dsrc, rptr, rcol, rsrc, lpos, lptr, lcol, lsrc;
 };
 
 // false when no diagonal qualifies (the format does not apply)
@@ -284,6 +289,13 @@ inline bool dia_layout(int n, const int *ptr, const int *col, DiaLayout &D)
     for (const auto &c : cand) D.off.push_back(c.second);
     std::sort(D.off.begin(), D.off.end());
     D.nd = (int)D.off.size();
+    D.runs.clear();
+    for (int d = 0; d < D.nd;) {
+        int m = 1;
+        while (d + m < D.nd && m < DIA_RUN_MAX && D.off[d + m] == D.off[d] + m) m++;
+        D.runs.push_back(d | (m << 8));
+        d += m;
+    }
     D.ldn = (n + DIA_ALIGN - 1) / DIA_ALIGN * DIA_ALIGN;
     // diagonal index by offset (reusing the count array): -1 for the others
     std::fill(cnt.begin(), cnt.end(), -1);

@@ -43,15 +43,15 @@ struct BEpiStore {
     __device__ void fin(SolveState *, const double *) const {}
 };
 
-template <class Y, class D>
-struct BEpiScale {  // y = d[row] * (A x)
+template <class Y, class D, int KB>
+struct BEpiScale {  // y = d[row] * (A x); element e of the block is row e / KB
+    static_assert(KB > 0 && (KB & (KB - 1)) == 0, "block width is a power of two");
     static constexpr int K = 0;
     static constexpr int XV = 0;  // d is an n-vector, counted separately
     Y *y;
     const D *d;
-    int kb;
     template <class S>
-    __device__ void row(size_t e, int, S s, double *) const { y[e] = (Y)d[e / kb] * (Y)s; }
+    __device__ void row(size_t e, int, S s, double *) const { y[e] = (Y)d[e / KB] * (Y)s; }
     __device__ void fin(SolveState *, const double *) const {}
 };
 

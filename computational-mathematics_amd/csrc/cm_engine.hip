@@ -1668,7 +1668,7 @@ struct BlockKrylov {
 
     void enqueue_apply(const T *in, T *out)
     {
-        spmm(*Hm, bH, in, BEpiScale<T, T>{tm, idv, KB}, st);
+        spmm(*Hm, bH, in, BEpiScale<T, T, KB>{tm, idv}, st);
         spmm(*G, bG, tm, BEpiStore<T>{out}, st);
     }
 

@@ -35,7 +35,7 @@ METRIC = "PBiCGSTAB solves/s + p99 latency at rel-res 1e-8; SpMV/M-apply HBM GB/
 from cmamd.workloads import WORKLOADS, build as build_workload, serve_requests, serve_systems  # noqa: E402
 
 CAT_NAMES = {"A": "A SpMV + fused dot(s)", "F1": "factor SpMV H (D^-1 fused)", "F2": "factor SpMV G",
-             "LU": "L/U SpMV (mode C)", "VEC": "BiCGSTAB vector updates", "SETUP": "setup r=b-Ax",
+             "LU": "L/U SpMV (mode C) or triangular solve (modes I/J)", "VEC": "BiCGSTAB vector updates", "SETUP": "setup r=b-Ax",
              "FIN": "one-workgroup reduction finalize"}
 CAT_KERNELS = {"A": "cm::spmv_kernel<2048, double, double, double, cm::EpiAlpha|EpiOmega<double> >",
                "F1": "cm::spmv_kernel<2048, double, double, double, cm::EpiScale<double, double> >",
@@ -230,6 +230,7 @@ def cpu_baseline(w, A, b, F, seconds):
     while True:
         t0 = time.perf_counter()
         x, _, it, info, _ = oracle.bicgstab(w["prec"], A, b, F, tol=w["tol"], nmax=w["nmax"],
+                                            nr=w.get("nr", 1), mode=w.get("mode", "E"),
                                             issym=w.get("sym", False))
         times.append(time.perf_counter() - t0)
         its.append(it)
@@ -699,9 +700,13 @@ def main():
 # The other configurations, run after the headline workload in compact form
 # (fewer steps and requests, short CPU baselines) so that a default run
 # reports every configuration in a few minutes.
-DEFAULT_LEGS = ("cd3d-128-pe5", "st27-100-rhs16", "serve-mix", "cd2d-100", "cd3d-128-sym")
+DEFAULT_LEGS = ("cd3d-128-pe5", "st27-100-rhs16", "serve-mix", "cd2d-100", "cd3d-128-sym", "cd3d-128-pe5-c2",
+                "cd3d-128-pe5-i", "cd3d-128-pe5-j")
 LEG_SETTINGS = {
     "cd3d-128-pe5": dict(steps=5, warmup=1, cpu_seconds=3.0),
+    "cd3d-128-pe5-c2": dict(steps=3, warmup=1, cpu_seconds=3.0),
+    "cd3d-128-pe5-i": dict(steps=3, warmup=1, cpu_seconds=3.0),
+    "cd3d-128-pe5-j": dict(steps=3, warmup=1, cpu_seconds=3.0),
     "cd3d-128-sym": dict(steps=5, warmup=1, cpu_seconds=3.0),
     "st27-100-rhs16": dict(steps=5, warmup=1, cpu_seconds=3.0),
     "serve-mix": dict(serve_requests=2000, serve_fp64_requests=300),
@@ -767,7 +772,7 @@ def bench_single(args, w):
     print(f"[bench] workload {args.workload}: n={A.n} nnz(A)={A_full.nnz} nnz(G)={F.G.nnz} "
           f"nnz(H)={nnz_h} factors built in {t_build:.1f}s", file=sys.stderr, flush=True)
 
-    sys_ = cmamd.System(w["prec"], A, F, a_symmetric=sym)
+    sys_ = cmamd.System(w["prec"], A, F, a_symmetric=sym, nr=w.get("nr", 1), mode=w.get("mode", "E"))
     if args.no_graphs:
         sys_.graphs(False)
     tuned = None
@@ -823,7 +828,8 @@ def bench_single(args, w):
     elif w["prec"] == "afiim":
         dr = cmamd.afiim_epbicgstab(A, b, F, tol=w["tol"], nmax=w["nmax"])
     else:
-        dr = cmamd.acifim_pbicgstab(A, b, F, tol=w["tol"], nmax=w["nmax"])
+        dr = cmamd.acifim_pbicgstab(A, b, F, nr=w.get("nr", 1), mode=w.get("mode", "E"), tol=w["tol"],
+                                    nmax=w["nmax"])
     t_dropin = time.perf_counter() - t0
     mixed = {}
     xs_mixed = {}
@@ -849,7 +855,13 @@ def bench_single(args, w):
     d_avg_us = 1e3 * ms / n_l
     d_gbs = by / ms / 1e6
     kname = CAT_KERNELS.get(dom, dom)
-    if tuned:
+    mode = w.get("mode", "E")
+    if dom == "LU":  # mode C's L / U products, or modes I / J's level-scheduled triangular solves
+        kname = ("cm::k_trsv_level<double, double> (one launch per dependency level)" if mode in ("I", "J")
+                 else "cm::spmv_kernel<2048, double, double, double, cm::EpiSubFrom<double> >")
+    elif mode == "C" and w.get("nr", 1) > 1 and dom in ("F1", "F2"):
+        kname = kname.split(", cm::Epi")[0] + ", cm::EpiStore|EpiAccum|EpiAccumScale<double> >"
+    if tuned and dom != "LU":
         mat = {"A": "A", "F1": "GT" if tuned["GT"] else "H", "F2": "G"}.get(dom)
         fmt = formats.get(64, {}).get(mat, "staged") if mat else "staged"
         if fmt.startswith("slot"):  # slot-major kernel: spmv_slot<V, X, Acc, Epi, 16-bit offsets>

@@ -26,6 +26,24 @@ WORKLOADS = {
         desc="3D 7-point convection-diffusion 128^3 (n=2097152, nnz=14581760), cell Peclet 5; "
              "ACIFIM(eps1=0.01, eps2=0.1), apply mode E; b=A x*, x0=0; plain FP64 BiCGSTAB to rel-res 1e-10",
         dim=3, N=128, pe=5.0, seed=4, xseed=5, prec="acifim", args=(0.01, 0.1), tol=1e-10, nmax=2000),
+    "cd3d-128-pe5-c2": dict(
+        desc="3D 7-point convection-diffusion 128^3 (n=2097152, nnz=14581760), cell Peclet 5; "
+             "ACIFIM(eps1=0.01, eps2=0.1), apply mode C with nr=2 (one Richardson correction with L and "
+             "one with U around the approximate inverses); b=A x*, x0=0; plain FP64 BiCGSTAB to rel-res 1e-10",
+        dim=3, N=128, pe=5.0, seed=4, xseed=5, prec="acifim", args=(0.01, 0.1), tol=1e-10, nmax=2000,
+        mode="C", nr=2),
+    "cd3d-128-pe5-i": dict(
+        desc="3D 7-point convection-diffusion 128^3 (n=2097152, nnz=14581760), cell Peclet 5; "
+             "ACIFIM(eps1=0.01, eps2=0.1), apply mode I (triangular solves with L and U, D^-1 between); "
+             "b=A x*, x0=0; plain FP64 BiCGSTAB to rel-res 1e-10",
+        dim=3, N=128, pe=5.0, seed=4, xseed=5, prec="acifim", args=(0.01, 0.1), tol=1e-10, nmax=2000,
+        mode="I"),
+    "cd3d-128-pe5-j": dict(
+        desc="3D 7-point convection-diffusion 128^3 (n=2097152, nnz=14581760), cell Peclet 5; "
+             "ACIFIM(eps1=0.01, eps2=0.1), apply mode J (triangular solves with L and U); "
+             "b=A x*, x0=0; plain FP64 BiCGSTAB to rel-res 1e-10",
+        dim=3, N=128, pe=5.0, seed=4, xseed=5, prec="acifim", args=(0.01, 0.1), tol=1e-10, nmax=2000,
+        mode="J"),
     "st27-100-rhs16": dict(
         desc="3D 27-point stencil 100^3 (n=1000000), off-diagonals -(1+0.3 U[-1,1]), diagonal 1.05 x row "
              "absolute sum, 5% of rows with 200-600 long-range couplings -0.01 U[0,1] (nnz~46.3M); AFIIM "

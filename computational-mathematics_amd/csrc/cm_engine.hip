@@ -269,6 +269,8 @@ struct DevCsr {
     bool lower = false;  // the level schedule's triangle (lower: rows read smaller columns)
     int *lrows = nullptr;
     std::vector<int> loff;
+    std::vector<long long> lnnz;  // per level: nonzeros of its rows
+    int lpr = 1;                  // lanes per row of the level kernel
     bool present() const { return ptr != nullptr; }
     // device bytes held (index arrays only when owned)
     size_t bytes() const
@@ -361,6 +363,8 @@ static int cast_csr(DevCsr<V> &dst, const DevCsr<double> &src, hipStream_t s)
     dst.lower = src.lower;
     dst.lrows = src.lrows;
     dst.loff = src.loff;
+    dst.lnnz = src.lnnz;
+    dst.lpr = src.lpr;
     CM_TRY(dev_alloc(&dst.val, (size_t)dst.nnz + CSR_PAD));
     CM_HIP(hipMemsetAsync(dst.val + dst.nnz, 0, CSR_PAD * sizeof(V), s));
     if constexpr (std::is_same<V, __half>::value) {
@@ -529,6 +533,13 @@ static int build_levels(DevCsr<double> &m, const int *ptr, const int *col, bool 
     m.nlev = L.nlev;
     m.lower = lower;
     m.loff = L.loff;
+    m.lnnz.assign(L.nlev, 0);  // entries each level's rows hold (read by its launch)
+    for (int l = 0; l < L.nlev; l++)
+        for (int q = L.loff[l]; q < L.loff[l + 1]; q++) m.lnnz[l] += ptr[L.rows[q] + 1] - ptr[L.rows[q]];
+    // lanes per row: the power of two at or above the mean stored row length, 1..16
+    const double mean = m.n ? (double)ptr[m.n] / m.n : 0.0;
+    m.lpr = 1;
+    while (m.lpr < 16 && m.lpr < mean) m.lpr *= 2;
     CM_TRY(dev_upload(&m.lrows, L.rows.data(), (size_t)m.n, s));
     CM_HIP(hipStreamSynchronize(s));
     return CM_OK;
@@ -877,10 +888,25 @@ struct Krylov {
     {
         for (int l = 0; l < m.nlev; l++) {
             const int cnt = m.loff[l + 1] - m.loff[l];
-            const int g = (cnt + BS - 1) / BS;
-            timed(KC_LU, (double)cnt * 3.0 * sizeof(T), [&] {
-                k_trsv_level<T, VF><<<g, BS, 0, stream>>>(cnt, m.lower, m.lrows + m.loff[l], m.ptr, m.col, m.val,
-                                                          m.rowscale, bin, xout, use_state ? st : nullptr);
+            // the level's entries (value + column), its row list and row pointers, and per row the
+            // right-hand side read, the solution written and (once over all levels) read back
+            const double by = (double)m.lnnz[l] * (sizeof(VF) + 4.0) + (double)cnt * (4.0 + 8.0 + 3.0 * sizeof(T));
+            timed(KC_LU, by, [&] {
+                const SolveState *gate = use_state ? st : nullptr;
+                const int *lr = m.lrows + m.loff[l];
+                auto go = [&](auto lanes) {
+                    constexpr int LPR = decltype(lanes)::value;
+                    const int g = (int)(((long long)cnt * LPR + BS - 1) / BS);
+                    k_trsv_level<T, VF, LPR><<<g, BS, 0, stream>>>(cnt, m.lower, lr, m.ptr, m.col, m.val, m.rowscale,
+                                                                   bin, xout, gate);
+                };
+                switch (m.lpr) {
+                case 1: go(std::integral_constant<int, 1>{}); break;
+                case 2: go(std::integral_constant<int, 2>{}); break;
+                case 4: go(std::integral_constant<int, 4>{}); break;
+                case 8: go(std::integral_constant<int, 8>{}); break;
+                default: go(std::integral_constant<int, 16>{}); break;
+                }
             });
         }
     }

@@ -4,40 +4,57 @@
 //
 // The host groups the rows into dependency levels once per factor (a row's
 // level is one more than the deepest row it reads); one launch per level then
-// solves every row of that level with one thread per row.  Rows of a level
-// are independent, so no workgroup ever waits on another and no kernel can
-// stall on a dependency: ordering comes from the stream.
+// solves every row of that level.  Rows of a level are independent, so no
+// workgroup ever waits on another and no kernel can stall on a dependency:
+// ordering comes from the stream.
 #pragma once
 #include "cm_kernels.hpp"
 
 namespace cm {
 
 // x[i] = (b[i] - sum_{j in triangle, j != i} T[i,j] x[j]) / T[i,i] for the
-// rows of one level.  Only the declared triangle is read (lower: j <= i,
-// upper: j >= i), as the TRIANGULAR descriptor makes mkl_sparse_d_trsv do
+// rows of one level (T[i,i] = 1 when the row stores no diagonal).  LPR lanes
+// per row (a power of two dividing the wave): lane l takes the row's entries
+// l, l + LPR, ..., so its gathers of x are issued together instead of one
+// after another, and a level of a few thousand rows still spreads over many
+// workgroups; the lanes' sums are added with xor shuffles in a fixed order.
+// Only the declared triangle is read (lower: j <= i, upper: j >= i), as the
+// TRIANGULAR descriptor makes mkl_sparse_d_trsv do
 // (ACIFIM/src/bicgstab.cpp:170-176); a stray entry on the other side is
 // ignored rather than read while another level writes it.
-template <class T, class V>
+template <class T, class V, int LPR>
 __global__ __launch_bounds__(BS) void k_trsv_level(int cnt, bool lower, const int *__restrict__ rows,
                                                    const int *__restrict__ ptr, const int *__restrict__ col,
                                                    const V *__restrict__ val, const float *__restrict__ rowscale,
                                                    const T *__restrict__ b, T *__restrict__ x,
                                                    const SolveState *st)
 {
+    static_assert(LPR >= 1 && LPR <= WAVE && (LPR & (LPR - 1)) == 0, "lanes per row: a power of two");
     if (st && st->done) return;
-    const int q = blockIdx.x * BS + threadIdx.x;
-    if (q >= cnt) return;
+    const int q = (int)((blockIdx.x * BS + threadIdx.x) / LPR), l = (int)(threadIdx.x % LPR);
+    if (q >= cnt) return;  // the row's LPR lanes leave together
     const int i = rows[q];
-    double s = (double)b[i], d = 1.0;
     const double sc = rowscale ? (double)rowscale[i] : 1.0;
-    for (int k = ptr[i]; k < ptr[i + 1]; k++) {
+    double s = 0.0, d = 0.0, hasd = 0.0;
+    const int e = ptr[i + 1];
+    for (int k = ptr[i] + l; k < e; k += LPR) {
         const int j = col[k];
         if (lower ? j > i : j < i) continue;
         const double a = load_val<V, double>(val, k) * sc;
-        if (j == i) d = a;
-        else s -= a * (double)x[j];
+        if (j == i) {
+            d = a;
+            hasd = 1.0;
+        } else {
+            s += a * (double)x[j];
+        }
     }
-    x[i] = (T)(s / d);
+#pragma unroll
+    for (int o = LPR / 2; o > 0; o >>= 1) {
+        s += __shfl_xor(s, o, LPR);
+        d += __shfl_xor(d, o, LPR);
+        hasd += __shfl_xor(hasd, o, LPR);
+    }
+    if (l == 0) x[i] = (T)(((double)b[i] - s) / (hasd > 0.0 ? d : 1.0));
 }
 
 }  // namespace cm

@@ -298,7 +298,7 @@ SOLVE_SYMBOLS = [
     "cm_system_enable_mixed", "cm_system_solve_mixed",
     "cm_system_block_set", "cm_system_block_get", "cm_system_block_solve", "cm_system_block_solve_mixed",
     "cm_system_graphs", "cm_system_block_graphs", "cm_system_block_release",
-    "cm_system_block_time_spmm", "cm_batch_solve", "cm_system_autotune", "cm_system_block_autotune",
+    "cm_system_block_time_spmm", "cm_system_dia_info", "cm_batch_solve", "cm_system_autotune", "cm_system_block_autotune",
     "cm_system_spmv_format", "cm_system_spmv_formats", "cm_system_profile_sampled", "cm_system_block_dia", "cm_last_error", "cm_device_count",
     "cm_server_create", "cm_server_configure", "cm_server_destroy", "cm_server_add_system", "cm_server_submit", "cm_server_wait",
     "cm_server_stats", "cm_batch_solve_noalloc", "cm_pdfiim_device",
@@ -343,6 +343,9 @@ def _type_solve(lib):
     lib.cm_system_graphs.argtypes = [vp, C.c_int]
     lib.cm_system_graphs.restype = C.c_int
     lib.cm_system_block_release.argtypes = [vp]
+    lib.cm_system_dia_info.argtypes = [vp, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int),
+                                       C.POINTER(C.c_longlong)]
+    lib.cm_system_dia_info.restype = C.c_int
     lib.cm_system_block_time_spmm.argtypes = [vp, C.c_char, C.c_int, C.c_int, PD, PD]
     lib.cm_system_block_time_spmm.restype = C.c_int
     lib.cm_batch_solve.argtypes = [C.c_int, C.POINTER(vp), C.POINTER(PD), C.POINTER(PD), PD, PI, C.c_int, PD,
@@ -537,6 +540,16 @@ class System:
         fmt in SPMV_FORMATS (EngineError when not available)."""
         _check(self._lib.cm_system_spmv_format(self._h, int(bits), self.MATRICES.index(which),
                                                self.SPMV_FORMATS.index(fmt)))
+
+    def dia_info(self, which="A"):
+        """(offsets of the stored diagonals, remainder entries) of matrix
+        'A', 'G', 'H' or 'GT' (the diagonal index is built if absent; no
+        offsets when no diagonal is held by 60% of the rows)."""
+        q = ("A", "G", "H", "GT").index(which)
+        nd, nrem = C.c_int(), C.c_longlong()
+        off = np.zeros(32, np.int32)
+        _check(self._lib.cm_system_dia_info(self._h, q, C.byref(nd), _ip(off), C.byref(nrem)))
+        return off[:nd.value].copy(), int(nrem.value)
 
     def spmv_formats(self):
         """{bits: {matrix: format}} of the resident copies."""

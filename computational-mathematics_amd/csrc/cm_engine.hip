@@ -29,14 +29,18 @@
 #include <algorithm>
 #include <climits>
 #include <cmath>
+#include <condition_variable>
 #include <cstdio>
 #include <cstring>
 #include <iostream>
 #include <memory>
 #include <mutex>
 #include <string>
+#include <thread>
 #include <type_traits>
 #include <vector>
+
+#include <hipcub/hipcub.hpp>
 
 #include "cm_kernels.hpp"
 #include "cm_solve.h"
@@ -113,6 +117,16 @@ __global__ __launch_bounds__(BS) void k_vmul(int n, const T *__restrict__ a, con
     for (int i = blockIdx.x * BS + threadIdx.x; i < n; i += gridDim.x * BS) z[i] = a[i] * d[i];
 }
 
+// Sets *bad to 1 when a stored column lies outside [0, n) (the upload's
+// column check, on the device).
+__global__ __launch_bounds__(BS) void k_col_range(long long nnz, const int *__restrict__ col, int n, int *bad)
+{
+    bool out = false;
+    for (long long k = (long long)blockIdx.x * BS + threadIdx.x; k < nnz; k += (long long)gridDim.x * BS)
+        out |= (unsigned)col[k] >= (unsigned)n;
+    if (out) *bad = 1;
+}
+
 // ------------------------------------------------------------ host helpers
 
 template <class T>
@@ -123,11 +137,112 @@ static int dev_alloc(T **d, size_t count)
     return CM_OK;
 }
 
+// Host-to-device copies of large arrays through pinned staging chunks: the
+// calling thread and STAGE_HELPERS helper threads copy the (pageable) source
+// into one chunk while the DMA engine moves the previous ones -- about twice
+// the rate of a pageable hipMemcpyAsync, which stages through the runtime's
+// own buffers with one thread.  One per process, made on first use and kept
+// until the process ends; uploads from several threads take turns.  Returns
+// once the source has been read; the device copies complete in the order of
+// the stream passed (a chunk is refilled only after the event recorded
+// behind its last copy, whichever upload that was).
+class Stager {
+  public:
+    static constexpr size_t CHUNK = (size_t)16 << 20, MIN_BYTES = (size_t)4 << 20;
+    static constexpr int NBUF = 4, STAGE_HELPERS = 3;
+    static Stager &get()
+    {
+        static Stager *s = new Stager();  // never destroyed: its helpers wait until the process ends
+        return *s;
+    }
+    int upload(void *dst, const void *src, size_t bytes, hipStream_t s)
+    {
+        std::lock_guard<std::mutex> lk(mu);
+        if (!buf[0]) {
+            for (auto &b : buf) CM_HIP(hipHostMalloc((void **)&b, CHUNK, hipHostMallocDefault));
+            for (int h = 0; h < STAGE_HELPERS; h++) std::thread([this] { help(); }).detach();
+        }
+        for (size_t off = 0; off < bytes; next = (next + 1) % NBUF) {
+            const int b = next;
+            const size_t len = std::min(CHUNK, bytes - off);
+            if (ev[b]) {  // the chunk's previous copy must have left it
+                const hipError_t e = hipEventSynchronize(ev[b]);
+                (void)hipEventDestroy(ev[b]);
+                ev[b] = nullptr;
+                if (e != hipSuccess) return fail(CM_ERR_HIP, "staged upload wait failed");
+            }
+            copy(buf[b], (const char *)src + off, len);
+            CM_HIP(hipMemcpyAsync((char *)dst + off, buf[b], len, hipMemcpyHostToDevice, s));
+            CM_HIP(hipEventCreateWithFlags(&ev[b], hipEventDisableTiming));  // on the stream's (current) device
+            CM_HIP(hipEventRecord(ev[b], s));
+            off += len;
+        }
+        return CM_OK;
+    }
+
+  private:
+    std::mutex mu;  // one upload at a time
+    char *buf[NBUF] = {};
+    hipEvent_t ev[NBUF] = {};  // recorded behind each chunk's last copy
+    int next = 0;              // the chunk filled next
+    struct Job {
+        char *d;
+        const char *s;
+        size_t n;
+    };
+    std::mutex jmu;
+    std::condition_variable jcv, dcv;
+    std::vector<Job> jobs;
+    int pending = 0;
+
+    // len bytes in STAGE_HELPERS + 1 parts, the first by the caller
+    void copy(char *d, const char *src, size_t len)
+    {
+        const size_t part = ((len + STAGE_HELPERS) / (STAGE_HELPERS + 1) + 4095) / 4096 * 4096;
+        {
+            std::lock_guard<std::mutex> lk(jmu);
+            for (size_t a = part; a < len; a += part) {
+                jobs.push_back({d + a, src + a, std::min(part, len - a)});
+                pending++;
+            }
+        }
+        jcv.notify_all();
+        std::memcpy(d, src, std::min(part, len));
+        std::unique_lock<std::mutex> lk(jmu);
+        dcv.wait(lk, [this] { return pending == 0; });
+    }
+    void help()
+    {
+        for (;;) {
+            Job j;
+            {
+                std::unique_lock<std::mutex> lk(jmu);
+                jcv.wait(lk, [this] { return !jobs.empty(); });
+                j = jobs.back();
+                jobs.pop_back();
+            }
+            std::memcpy(j.d, j.s, j.n);
+            std::lock_guard<std::mutex> lk(jmu);
+            if (--pending == 0) dcv.notify_all();
+        }
+    }
+};
+
+// count T from host memory h into device memory d (staged when large)
+template <class T>
+static int copy_h2d(T *d, const T *h, size_t count, hipStream_t s)
+{
+    const size_t bytes = count * sizeof(T);
+    if (bytes >= Stager::MIN_BYTES) return Stager::get().upload(d, h, bytes, s);
+    CM_HIP(hipMemcpyAsync(d, h, bytes, hipMemcpyHostToDevice, s));
+    return CM_OK;
+}
+
 template <class T>
 static int dev_upload(T **d, const T *h, size_t count, hipStream_t s)
 {
     CM_TRY(dev_alloc(d, count));
-    if (h && count) CM_HIP(hipMemcpyAsync(*d, h, count * sizeof(T), hipMemcpyHostToDevice, s));
+    if (h && count) CM_TRY(copy_h2d(*d, h, count, s));
     return CM_OK;
 }
 
@@ -137,7 +252,7 @@ template <class T>
 static int dev_upload_padded(T **d, const T *h, size_t count, hipStream_t s)
 {
     CM_TRY(dev_alloc(d, count + CSR_PAD));
-    if (h && count) CM_HIP(hipMemcpyAsync(*d, h, count * sizeof(T), hipMemcpyHostToDevice, s));
+    if (h && count) CM_TRY(copy_h2d(*d, h, count, s));
     CM_HIP(hipMemsetAsync(*d + count, 0, CSR_PAD * sizeof(T), s));
     return CM_OK;
 }
@@ -322,17 +437,38 @@ static int check_csr(int n, const int *ptr, const int *col)
     return CM_OK;
 }
 
-static int upload_csr(DevCsr<double> &m, int n, const int *ptr, const int *col, const double *val, hipStream_t s)
+// Uploads a CSR matrix.  bad: null to check the columns on the host (for
+// callers that go on reading them there), else a device flag that the
+// columns' range check sets, which the caller reads back before any kernel
+// uses the matrix.
+static int upload_csr(DevCsr<double> &m, int n, const int *ptr, const int *col, const double *val, hipStream_t s,
+                      int *bad = nullptr)
 {
-    CM_TRY(check_csr(n, ptr, col));
+    if (!bad) {
+        CM_TRY(check_csr(n, ptr, col));
+    } else if (const char *why = csr_ptr_problem(n, ptr, col)) {
+        return fail(CM_ERR_ARG, why);
+    }
     m.release();
     m.owns_index = true;
     m.n = n;
     m.nnz = ptr[n];
-    std::vector<int> blk = block_table(n, ptr, &m.nblk);
+    // the staged row blocks are cut on another thread while the arrays upload
+    std::vector<int> blk;
+    int nblk = 0;
+    struct Joined {
+        std::thread t;
+        ~Joined()
+        {
+            if (t.joinable()) t.join();
+        }
+    } cut{std::thread([&] { blk = block_table(n, ptr, &nblk); })};
     CM_TRY(dev_upload(&m.ptr, ptr, (size_t)n + 1, s));
     CM_TRY(dev_upload_padded(&m.col, col, (size_t)m.nnz, s));
+    if (bad && m.nnz) k_col_range<<<long_grid(m.nnz), BS, 0, s>>>(m.nnz, m.col, n, bad);
     CM_TRY(dev_upload_padded(&m.val, val, (size_t)m.nnz, s));
+    cut.t.join();
+    m.nblk = nblk;
     CM_TRY(dev_upload(&m.blk, blk.data(), blk.size(), s));
     CM_HIP(hipStreamSynchronize(s));  // blk is a host temporary
     return CM_OK;
@@ -458,13 +594,229 @@ static int build_slot_vals(DevCsr<V> &m, hipStream_t s)
     return CM_OK;
 }
 
+// ------------------------------------------- diagonal index on the device
+//
+// The layout dia_layout computes on the host, built from the resident
+// matrix without copying it back: the candidate offsets are those of
+// DIA_SAMPLE_ROWS sample rows spread by a multiplicative hash (row s * K mod
+// n, K odd: distinct rows when n is a power of two, and no stride that lines
+// up with a grid's boundary rows, which evenly spaced rows of a 2^k-row grid
+// all are), so a diagonal held by 60% of the rows is missed only if its gaps
+// fall on every sampled row; their
+// entries are counted exactly (a histogram in LDS per workgroup), the
+// selection follows dia_layout's rule, and the diagonal positions and the
+// CSR remainder are placed by one thread per row in the entries' order.
+// Matrices with a sampled row longer than DIA_SAMPLE_LEN, more than
+// DIA_CAND_MAX candidate offsets, or remainders longer than SHORT_MAX (whose
+// block products need the long-row tiles) are left to the host.
+constexpr int DIA_SAMPLE_ROWS = 1024, DIA_SAMPLE_LEN = 64, DIA_CAND_MAX = 1024;
+
+// offsets (column - row) of the S sample rows, DIA_SAMPLE_LEN slots per row
+// (INT_MIN where the row ends); *over set when a sample row is longer
+__global__ __launch_bounds__(BS) void k_dia_sample(int n, int S, const int *__restrict__ ptr,
+                                                   const int *__restrict__ col, int *out, int *over)
+{
+    for (int q = blockIdx.x * BS + threadIdx.x; q < S * DIA_SAMPLE_LEN; q += gridDim.x * BS) {
+        const int smp = q / DIA_SAMPLE_LEN, j = q % DIA_SAMPLE_LEN;
+        const int i = S == n ? smp : (int)((unsigned long long)smp * 2654435761ULL % (unsigned long long)n);
+        const int b = ptr[i], e = ptr[i + 1];
+        out[q] = b + j < e ? col[b + j] - i : INT_MIN;
+        if (j == 0 && e - b > DIA_SAMPLE_LEN) *over = 1;
+    }
+}
+
+__global__ __launch_bounds__(BS) void k_fill_int(long long cnt, int v, int *a)
+{
+    for (long long q = (long long)blockIdx.x * BS + threadIdx.x; q < cnt; q += (long long)gridDim.x * BS) a[q] = v;
+}
+
+// map[list[q] + n - 1] = q
+__global__ __launch_bounds__(BS) void k_dia_mark(int c, const int *__restrict__ list, int n, int *map)
+{
+    for (int q = blockIdx.x * BS + threadIdx.x; q < c; q += gridDim.x * BS) map[(size_t)(list[q] + n - 1)] = q;
+}
+
+// entries on each of the c marked offsets: LDS histogram per workgroup
+__global__ __launch_bounds__(BS) void k_dia_hist(int n, const int *__restrict__ ptr, const int *__restrict__ col,
+                                                 const int *__restrict__ map, int c, int *cnt)
+{
+    __shared__ int h[DIA_CAND_MAX];
+    for (int q = threadIdx.x; q < c; q += BS) h[q] = 0;
+    __syncthreads();
+    for (int i = blockIdx.x * BS + threadIdx.x; i < n; i += gridDim.x * BS)
+        for (int k = ptr[i]; k < ptr[i + 1]; k++) {
+            const int q = map[(size_t)(col[k] - i + n - 1)];
+            if (q >= 0) atomicAdd(&h[q], 1);
+        }
+    __syncthreads();
+    for (int q = threadIdx.x; q < c; q += BS)
+        if (h[q]) atomicAdd(&cnt[q], h[q]);
+}
+
+// dsrc[d * ldn + i] = k for the entries on diagonal d = map[offset] (the
+// last one when a row repeats a column, as on the host); rcnt[i] = the row's
+// other entries; *longrem set when a row keeps more than SHORT_MAX of them
+__global__ __launch_bounds__(BS) void k_dia_place(int n, int ldn, const int *__restrict__ ptr,
+                                                  const int *__restrict__ col, const int *__restrict__ map,
+                                                  int *dsrc, int *rcnt, int *longrem)
+{
+    for (int i = blockIdx.x * BS + threadIdx.x; i < n; i += gridDim.x * BS) {
+        int r = 0;
+        for (int k = ptr[i]; k < ptr[i + 1]; k++) {
+            const int d = map[(size_t)(col[k] - i + n - 1)];
+            if (d >= 0)
+                dsrc[(size_t)d * ldn + i] = k;
+            else
+                r++;
+        }
+        rcnt[i] = r;
+        if (r > SHORT_MAX) *longrem = 1;
+    }
+}
+
+// the remainder entries of every row, in order, from rptr[i]
+__global__ __launch_bounds__(BS) void k_dia_rem(int n, const int *__restrict__ ptr, const int *__restrict__ col,
+                                                const int *__restrict__ map, const int *__restrict__ rptr, int *rcol,
+                                                int *rsrc)
+{
+    for (int i = blockIdx.x * BS + threadIdx.x; i < n; i += gridDim.x * BS) {
+        int pos = rptr[i];
+        for (int k = ptr[i]; k < ptr[i + 1]; k++)
+            if (map[(size_t)(col[k] - i + n - 1)] < 0) {
+                rcol[pos] = col[k];
+                rsrc[pos] = k;
+                pos++;
+            }
+    }
+}
+
+// Device temporaries freed together at scope exit.
+struct DevTemps {
+    std::vector<void *> p;
+    ~DevTemps()
+    {
+        for (void *q : p) (void)hipFree(q);
+    }
+    template <class T>
+    int alloc(T **d, size_t cnt)
+    {
+        CM_TRY(dev_alloc(d, cnt));
+        p.push_back(*d);
+        return CM_OK;
+    }
+};
+
+// Device build of m's diagonal index; *done false when the matrix is left to
+// the host build.  m.dia stays absent (done) when no diagonal qualifies.
+static int build_dia_device(DevCsr<double> &m, hipStream_t s, bool *done)
+{
+    *done = false;
+    const int n = m.n;
+    if (n < 1 || m.nnz < 1 || 2LL * n - 1 > INT_MAX) return CM_OK;
+    DevTemps tmp;
+    const int S = std::min(n, DIA_SAMPLE_ROWS);
+    int *smp, *flags;  // flags: [0] a long sample row, [1] a long remainder
+    CM_TRY(tmp.alloc(&smp, (size_t)S * DIA_SAMPLE_LEN));
+    CM_TRY(tmp.alloc(&flags, 2));
+    CM_HIP(hipMemsetAsync(flags, 0, 2 * sizeof(int), s));
+    k_dia_sample<<<(S * DIA_SAMPLE_LEN + BS - 1) / BS, BS, 0, s>>>(n, S, m.ptr, m.col, smp, flags);
+    std::vector<int> hs((size_t)S * DIA_SAMPLE_LEN);
+    int hf[2] = {0, 0};
+    CM_HIP(hipMemcpyAsync(hs.data(), smp, hs.size() * sizeof(int), hipMemcpyDeviceToHost, s));
+    CM_HIP(hipMemcpyAsync(hf, flags, 2 * sizeof(int), hipMemcpyDeviceToHost, s));
+    CM_HIP(hipStreamSynchronize(s));
+    if (hf[0]) return CM_OK;
+    std::vector<int> cand;
+    for (int o : hs)
+        if (o != INT_MIN) cand.push_back(o);
+    std::sort(cand.begin(), cand.end());
+    cand.erase(std::unique(cand.begin(), cand.end()), cand.end());
+    const int c = (int)cand.size();
+    if (c == 0 || c > DIA_CAND_MAX) return CM_OK;
+    // exact counts of the candidates
+    const size_t nmap = 2 * (size_t)n - 1;
+    int *map, *dl, *cnt;
+    CM_TRY(tmp.alloc(&map, nmap));
+    CM_TRY(tmp.alloc(&dl, (size_t)c));
+    CM_TRY(tmp.alloc(&cnt, (size_t)c));
+    k_fill_int<<<long_grid((long long)nmap), BS, 0, s>>>((long long)nmap, -1, map);
+    CM_HIP(hipMemcpyAsync(dl, cand.data(), (size_t)c * sizeof(int), hipMemcpyHostToDevice, s));
+    CM_HIP(hipMemsetAsync(cnt, 0, (size_t)c * sizeof(int), s));
+    k_dia_mark<<<(c + BS - 1) / BS, BS, 0, s>>>(c, dl, n, map);
+    k_dia_hist<<<std::min((n + BS - 1) / BS, 1024), BS, 0, s>>>(n, m.ptr, m.col, map, c, cnt);
+    std::vector<int> hc((size_t)c);
+    CM_HIP(hipMemcpyAsync(hc.data(), cnt, (size_t)c * sizeof(int), hipMemcpyDeviceToHost, s));
+    CM_HIP(hipStreamSynchronize(s));
+    // dia_layout's selection among the candidates
+    std::vector<std::pair<long long, int>> counted;
+    for (int q = 0; q < c; q++) counted.push_back({hc[q], cand[q]});
+    const std::vector<int> off = dia_select(n, counted);
+    *done = true;
+    if (off.empty()) return CM_OK;
+    const std::vector<int> runs = dia_runs(off);
+    const int nd = (int)off.size(), ldn = (n + DIA_ALIGN - 1) / DIA_ALIGN * DIA_ALIGN;
+    auto D = std::make_shared<DiaIndex>();
+    D->n = n;
+    D->ldn = ldn;
+    D->nd = nd;
+    D->nrun = (int)runs.size();
+    CM_TRY(dev_upload(&D->off, off.data(), off.size(), s));
+    CM_TRY(dev_upload(&D->runs, runs.data(), runs.size(), s));
+    // positions on the selected diagonals (map: offset -> diagonal), remainder counts
+    k_fill_int<<<long_grid((long long)nmap), BS, 0, s>>>((long long)nmap, -1, map);
+    k_dia_mark<<<(nd + BS - 1) / BS, BS, 0, s>>>(nd, D->off, n, map);
+    CM_TRY(dev_alloc(&D->dsrc, (size_t)nd * ldn));
+    k_fill_int<<<long_grid((long long)nd * ldn), BS, 0, s>>>((long long)nd * ldn, -1, D->dsrc);
+    int *rcnt, *rptr;
+    CM_TRY(tmp.alloc(&rcnt, (size_t)n));
+    CM_TRY(tmp.alloc(&rptr, (size_t)n + 1));
+    CM_HIP(hipMemsetAsync(rptr, 0, sizeof(int), s));
+    const int gr = std::min((n + BS - 1) / BS, 8192);
+    k_dia_place<<<gr, BS, 0, s>>>(n, ldn, m.ptr, m.col, map, D->dsrc, rcnt, flags + 1);
+    size_t tb = 0;
+    CM_HIP(hipcub::DeviceScan::InclusiveSum(nullptr, tb, rcnt, rptr + 1, n, s));
+    char *tbuf;
+    CM_TRY(tmp.alloc(&tbuf, std::max<size_t>(tb, 1)));
+    CM_HIP(hipcub::DeviceScan::InclusiveSum((void *)tbuf, tb, rcnt, rptr + 1, n, s));
+    int tot = 0;
+    CM_HIP(hipMemcpyAsync(&tot, rptr + n, sizeof(int), hipMemcpyDeviceToHost, s));
+    CM_HIP(hipMemcpyAsync(hf, flags, 2 * sizeof(int), hipMemcpyDeviceToHost, s));
+    CM_HIP(hipStreamSynchronize(s));
+    if (hf[1]) {  // long remainders: the host build cuts them into tiles
+        *done = false;
+        return CM_OK;
+    }
+    D->nrem = tot;
+    // no long rows: the tile fields as dia_layout leaves them
+    D->nlong = 0;
+    D->ntile = std::min(LONG_TILES_MAX, std::max(1, (n + LONG_TILE_COLS - 1) / LONG_TILE_COLS));
+    D->tw = (n + D->ntile - 1) / D->ntile;
+    D->nseg = 0;
+    D->nlrem = 0;
+    if (tot > 0) {
+        D->rptr = rptr;
+        tmp.p.erase(std::find(tmp.p.begin(), tmp.p.end(), (void *)rptr));  // kept by the index
+        CM_TRY(dev_alloc(&D->rcol, (size_t)tot));
+        CM_TRY(dev_alloc(&D->rsrc, (size_t)tot));
+        k_dia_rem<<<gr, BS, 0, s>>>(n, m.ptr, m.col, map, D->rptr, D->rcol, D->rsrc);
+    }
+    CM_HIP(hipGetLastError());
+    CM_HIP(hipStreamSynchronize(s));  // before the temporaries are freed
+    m.dia = D;
+    return CM_OK;
+}
+
 // Diagonal index of a resident FP64 matrix (spmv_dia): the diagonals held
 // by at least DIA_FILL of the rows (the DIA_MAX most populated, in offset
 // order), the other entries as a CSR remainder.  Left absent when no
-// diagonal qualifies.
+// diagonal qualifies.  Built on the device where build_dia_device takes the
+// matrix, else from a host copy (dia_layout).
 static int build_dia(DevCsr<double> &m, hipStream_t s)
 {
     if (!m.present() || m.dia) return CM_OK;
+    bool done = false;
+    CM_TRY(build_dia_device(m, s, &done));
+    if (done) return CM_OK;
     const int n = m.n;
     std::vector<int> ptr(n + 1), col(m.nnz);
     CM_HIP(hipMemcpyAsync(ptr.data(), m.ptr, ptr.size() * sizeof(int), hipMemcpyDeviceToHost, s));
@@ -1134,7 +1486,17 @@ static int create_k64(K64 &k, const HostSys &h, hipStream_t strm)
     const char mode = (h.variant == 2) ? h.mode : 'E';
     k.copy_shape(h.variant, h.n, h.nr, mode, h.variant == 1 && h.a_symmetric, strm);
     const int n = h.n;
-    if (k.sym) {
+    // column ranges of the matrices whose columns the host does not read
+    // again: checked on the device, one flag read back at the end
+    struct Flag {
+        int *d = nullptr;
+        ~Flag() { (void)hipFree(d); }
+    } bad;
+    CM_TRY(dev_alloc(&bad.d, 1));
+    CM_HIP(hipMemsetAsync(bad.d, 0, sizeof(int), strm));
+    if (k.sym) {  // the expansion and the transpose read the columns on the host
+        CM_TRY(check_csr(n, h.Ai, h.Aj));
+        CM_TRY(check_csr(n, h.Gi, h.Gj));
         std::vector<int> fp, fc;
         std::vector<double> fv;
         host_sym_expand(n, h.Ai, h.Aj, h.Av, fp, fc, fv);
@@ -1144,21 +1506,25 @@ static int create_k64(K64 &k, const HostSys &h, hipStream_t strm)
         host_transpose(n, h.Gi, h.Gj, h.Gv, tp, tc, tv);
         CM_TRY(upload_csr(k.GT, n, tp.data(), tc.data(), tv.data(), strm));
     } else {
-        CM_TRY(upload_csr(k.A, n, h.Ai, h.Aj, h.Av, strm));
-        if (k.uses_gh()) CM_TRY(upload_csr(k.H, n, h.Hi, h.Hj, h.Hv, strm));
+        CM_TRY(upload_csr(k.A, n, h.Ai, h.Aj, h.Av, strm, bad.d));
+        if (k.uses_gh()) CM_TRY(upload_csr(k.H, n, h.Hi, h.Hj, h.Hv, strm, bad.d));
     }
-    if (k.uses_gh()) CM_TRY(upload_csr(k.G, n, h.Gi, h.Gj, h.Gv, strm));
+    if (k.uses_gh()) CM_TRY(upload_csr(k.G, n, h.Gi, h.Gj, h.Gv, strm, k.sym ? nullptr : bad.d));
     if (k.uses_lu()) {
-        CM_TRY(upload_csr(k.L, n, h.Li, h.Lj, h.Lv, strm));
-        CM_TRY(upload_csr(k.U, n, h.Ui, h.Uj, h.Uv, strm));
-        if (k.mode == 'I' || k.mode == 'J') {
+        const bool levels = k.mode == 'I' || k.mode == 'J';  // the level schedule reads the columns on the host
+        CM_TRY(upload_csr(k.L, n, h.Li, h.Lj, h.Lv, strm, levels ? nullptr : bad.d));
+        CM_TRY(upload_csr(k.U, n, h.Ui, h.Uj, h.Uv, strm, levels ? nullptr : bad.d));
+        if (levels) {
             CM_TRY(build_levels(k.L, h.Li, h.Lj, true, strm));
             CM_TRY(build_levels(k.U, h.Ui, h.Uj, false, strm));
         }
     }
     CM_TRY(dev_upload(&k.idv, h.IDv, (size_t)n, strm));
     CM_TRY(k.alloc_work(k.A.nblk));
+    int hbad = 0;
+    CM_HIP(hipMemcpyAsync(&hbad, bad.d, sizeof(int), hipMemcpyDeviceToHost, strm));
     CM_HIP(hipStreamSynchronize(strm));
+    if (hbad) return fail(CM_ERR_ARG, CSR_COL_RANGE);
     return CM_OK;
 }
 
@@ -2836,6 +3202,24 @@ int cm_system_spmv_format(cm_system h, int bits, int which, int format)
     });
 }
 
+int cm_system_dia_info(cm_system h, int which, int *nd, int *offsets, long long *nrem)
+{
+    if (!h || !nd || !offsets || !nrem) return fail(CM_ERR_ARG, "null argument");
+    if (which < 0 || which > 3) return fail(CM_ERR_ARG, "matrix index must be 0..3 (A, G, H, GT)");
+    K64 &k = *h->sys.k64;
+    DevCsr<double> *ms[] = {&k.A, &k.G, &k.H, &k.GT};
+    DevCsr<double> &m = *ms[which];
+    *nd = 0;
+    *nrem = 0;
+    if (!m.present()) return fail(CM_ERR_ARG, "matrix not resident");
+    CM_TRY(build_dia(m, k.stream));
+    if (!m.dia) return CM_OK;
+    *nd = m.dia->nd;
+    *nrem = m.dia->nrem;
+    CM_HIP(hipMemcpy(offsets, m.dia->off, (size_t)m.dia->nd * sizeof(int), hipMemcpyDeviceToHost));
+    return CM_OK;
+}
+
 int cm_system_spmv_formats(cm_system h, int *formats)
 {
     if (!h || !formats) return fail(CM_ERR_ARG, "bad argument");
@@ -3354,6 +3738,60 @@ int cm_system_kernel_format_bytes(cm_system h, int category, double *total_bytes
     return CM_OK;
 }
 
+// Formats of a one-shot system without timing anything: the diagonal format
+// (built on the device) for each SpMV matrix whose diagonals held by 60% of
+// the rows leave at most ONESHOT_DIA_REM of its nonzeros to the remainder --
+// the second variant from ONESHOT_DIA2_ND diagonals up, where it times
+// faster -- the staged row blocks otherwise.
+constexpr double ONESHOT_DIA_REM = 0.3;
+constexpr int ONESHOT_DIA2_ND = 12;
+static int oneshot_formats(System &S)
+{
+    K64 &k = *S.k64;
+    for (DevCsr<double> *m : {&k.A, &k.G, &k.H, &k.GT}) {
+        if (!m->present()) continue;
+        CM_TRY(build_dia(*m, k.stream));
+        if (!m->dia || (double)m->dia->nrem > ONESHOT_DIA_REM * (double)m->nnz) continue;
+        CM_TRY(set_format(*m, m->dia->nd >= ONESHOT_DIA2_ND ? FMT_DIA2 : FMT_DIA, k.stream));
+        CM_TRY(k.ensure_partials((m->n + BS - 1) / BS));
+    }
+    return CM_OK;
+}
+
+// One-shot systems are freed off the caller's thread: the hipFree calls for
+// a system's few dozen buffers take ~10 ms that the call need not wait for.
+// One at a time: the next one-shot call (and library unload) first waits
+// for the previous release, so at most one released system is held.
+class Releaser {
+  public:
+    void hand(System *s)
+    {
+        std::lock_guard<std::mutex> lk(mu);
+        wait_locked();
+        int dev = 0;
+        (void)hipGetDevice(&dev);
+        th = std::thread([s, dev] {
+            (void)hipSetDevice(dev);
+            delete s;
+        });
+    }
+    void wait()
+    {
+        std::lock_guard<std::mutex> lk(mu);
+        wait_locked();
+    }
+    ~Releaser() { wait(); }
+
+  private:
+    std::mutex mu;
+    std::thread th;
+    void wait_locked()
+    {
+        if (th.joinable()) th.join();
+    }
+};
+static Releaser g_oneshot_release;
+
 static int dropin(int variant, int n, int a_symmetric, const int *Ai, const int *Aj, const double *Av,
                   const double *b, double tol, int NMAX, const int *Gi, const int *Gj, const double *Gv,
                   const double *IDv, const int *Hi, const int *Hj, const double *Hv, const int *Li, const int *Lj,
@@ -3362,10 +3800,16 @@ static int dropin(int variant, int n, int a_symmetric, const int *Ai, const int 
 {
     if (!b || !x || !resval || !iter || !info) return fail(CM_ERR_ARG, "null argument");
     if (cm_device_count() <= 0) return fail(CM_ERR_NODEV, "no GPU visible");
-    System S;
+    g_oneshot_release.wait();
+    std::unique_ptr<System> S(new System());
     const HostSys hs{variant, n, a_symmetric, nr, mode, Ai, Aj, Av, Gi, Gj, Gv, IDv, Hi, Hj, Hv, Li, Lj, Lv, Ui, Uj, Uv};
-    CM_TRY(system_create(S, hs));
-    return dropin_solve(S, b, x, tol, NMAX, resval, iter, verbose, info);
+    int rc = system_create(*S, hs);
+    if (rc == CM_OK) rc = oneshot_formats(*S);
+    if (rc == CM_OK) rc = dropin_solve(*S, b, x, tol, NMAX, resval, iter, verbose, info);
+    if (rc == CM_OK) {
+        g_oneshot_release.hand(S.release());
+    }
+    return rc;  // on an error the system is freed here, with the error's message kept
 }
 
 int cm_afiim_epbicgstab(int n, double *Av, int *Aj, int *Ai, double *b, double tol, int NMAX, double *Gv, int *Gj,

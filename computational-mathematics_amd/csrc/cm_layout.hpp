@@ -35,15 +35,25 @@ static_assert(DIA_MAX < 256, "a run stores its first diagonal in 8 bits");
 constexpr int LONG_TILE_COLS = 32768, LONG_TILES_MAX = 32, LONG_SEG_MIN = 8;
 
 // Why a CSR matrix would make a kernel read outside its arrays, or null.
-inline const char *csr_problem(int n, const int *ptr, const int *col)
+// The row-pointer half of csr_problem (the column range is checked apart,
+// on the device, when the columns are not read on the host).
+inline const char *csr_ptr_problem(int n, const int *ptr, const int *col)
 {
     if (n < 0 || !ptr) return "null CSR row pointer or negative size";
     if (ptr[0] != 0) return "CSR row pointer must start at 0";
     for (int i = 0; i < n; i++)
         if (ptr[i + 1] < ptr[i]) return "CSR row pointer not monotone";
     if (ptr[n] > 0 && !col) return "null CSR column array";
+    return nullptr;
+}
+
+constexpr const char *CSR_COL_RANGE = "CSR column index out of range";
+
+inline const char *csr_problem(int n, const int *ptr, const int *col)
+{
+    if (const char *why = csr_ptr_problem(n, ptr, col)) return why;
     for (long long k = 0; k < ptr[n]; k++)
-        if (col[k] < 0 || col[k] >= n) return "CSR column index out of range";
+        if (col[k] < 0 || col[k] >= n) return CSR_COL_RANGE;
     return nullptr;
 }
 
@@ -269,6 +279,39 @@ struct DiaLayout {
     std::vector<int> off, runs, dsrc, rptr, rcol, rsrc, lpos, lptr, lcol, lsrc;
 };
 
+// The stored diagonals from (entry count, offset) pairs: those held by at
+// least DIA_FILL of the n rows, the DIA_MAX most populated (ties: the
+// smaller offset), in offset order.
+inline std::vector<int> dia_select(int n, std::vector<std::pair<long long, int>> cand)
+{
+    cand.erase(std::remove_if(cand.begin(), cand.end(),
+                              [n](const std::pair<long long, int> &c) { return !((double)c.first >= DIA_FILL * n); }),
+               cand.end());
+    std::sort(cand.begin(), cand.end(), [](const auto &a, const auto &b) {
+        return a.first != b.first ? a.first > b.first : a.second < b.second;
+    });
+    if ((int)cand.size() > DIA_MAX) cand.resize(DIA_MAX);
+    std::vector<int> off;
+    for (const auto &c : cand) off.push_back(c.second);
+    std::sort(off.begin(), off.end());
+    return off;
+}
+
+// Runs of consecutive offsets (ascending off), at most DIA_RUN_MAX long:
+// first diagonal | length << 8.
+inline std::vector<int> dia_runs(const std::vector<int> &off)
+{
+    std::vector<int> runs;
+    const int nd = (int)off.size();
+    for (int d = 0; d < nd;) {
+        int m = 1;
+        while (d + m < nd && m < DIA_RUN_MAX && off[d + m] == off[d] + m) m++;
+        runs.push_back(d | (m << 8));
+        d += m;
+    }
+    return runs;
+}
+
 // false when no diagonal qualifies (the format does not apply)
 inline bool dia_layout(int n, const int *ptr, const int *col, DiaLayout &D)
 {
@@ -279,23 +322,11 @@ inline bool dia_layout(int n, const int *ptr, const int *col, DiaLayout &D)
         for (int k = ptr[i]; k < ptr[i + 1]; k++) cnt[(size_t)(col[k] - i + n - 1)]++;
     std::vector<std::pair<long long, int>> cand;
     for (size_t o = 0; o < cnt.size(); o++)
-        if ((double)cnt[o] >= DIA_FILL * n) cand.push_back({cnt[o], (int)((long long)o - (n - 1))});
-    if (cand.empty()) return false;
-    std::sort(cand.begin(), cand.end(), [](const auto &a, const auto &b) {
-        return a.first != b.first ? a.first > b.first : a.second < b.second;
-    });
-    if ((int)cand.size() > DIA_MAX) cand.resize(DIA_MAX);
-    D.off.clear();
-    for (const auto &c : cand) D.off.push_back(c.second);
-    std::sort(D.off.begin(), D.off.end());
+        if (cnt[o] > 0) cand.push_back({cnt[o], (int)((long long)o - (n - 1))});
+    D.off = dia_select(n, std::move(cand));
+    if (D.off.empty()) return false;
     D.nd = (int)D.off.size();
-    D.runs.clear();
-    for (int d = 0; d < D.nd;) {
-        int m = 1;
-        while (d + m < D.nd && m < DIA_RUN_MAX && D.off[d + m] == D.off[d] + m) m++;
-        D.runs.push_back(d | (m << 8));
-        d += m;
-    }
+    D.runs = dia_runs(D.off);
     D.ldn = (n + DIA_ALIGN - 1) / DIA_ALIGN * DIA_ALIGN;
     // diagonal index by offset (reusing the count array): -1 for the others
     std::fill(cnt.begin(), cnt.end(), -1);

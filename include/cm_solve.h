@@ -217,6 +217,12 @@ int cm_system_spmv_format(cm_system sys, int bits, int which, int format);
 /* Current formats: formats[18] = {FP64, FP32, FP16} x {A, G, H, G^T, L, U},
  * -1 for an absent matrix or a precision not enabled. */
 int cm_system_spmv_formats(cm_system sys, int *formats);
+/* Diagonal index of one resident matrix (which 0..3 = A, G, H, G^T), built
+ * if absent: *nd stored diagonals (0 when none is held by 60% of the rows),
+ * their offsets (column - row, ascending) in offsets[0..*nd) (room for 32
+ * needed), and *nrem entries left to the CSR remainder.  No counterpart in
+ * the reference (a view of this engine's storage choice). */
+int cm_system_dia_info(cm_system sys, int which, int *nd, int *offsets, long long *nrem);
 /* Autotuner of the current block's SpMM row binning (bits 64, 32 or 16):
  * for A, G and H the short-row limit among 16, 32, 64, 128 nonzeros that
  * times fastest; short_max[3] receives the choices. */

@@ -1,6 +1,10 @@
-"""Error behaviour at the C ABI on a GPU: malformed inputs are rejected on the
-host, before any kernel runs, with the documented codes (CM_ERR_ARG = -2,
-CM_ERR_UNSUPPORTED = -3) and a message; the engine stays usable afterwards.
+"""Error behaviour at the C ABI on a GPU: malformed inputs are rejected
+before any solve kernel reads them, with the documented codes (CM_ERR_ARG =
+-2, CM_ERR_UNSUPPORTED = -3) and a message; the engine stays usable
+afterwards.  Row pointers are checked on the host; column ranges on the host
+where the host reads the columns itself (symmetric expansion, transposes,
+level schedules), else by a check kernel on the device before the system is
+returned.
 The reference itself has no error returns (its only failure is
 std::bad_alloc); the drop-in C++ header turns these codes into
 std::runtime_error."""
@@ -34,6 +38,39 @@ def test_bad_csr_rejected():
     # the engine still works
     s = cmamd.System("afiim", A, F)
     s.set_rhs(cmamd.csr_mv_host(A, np.ones(A.n)))
+    s.set_x(None)
+    assert s.solve(tol=1e-8, nmax=200)[2] == 0
+    s.close()
+
+
+def _bad_col(M, value, at=5):
+    bad = cmamd.Csr(M.n, M.ptr.copy(), M.idx.copy(), M.val.copy())
+    bad.idx[at] = value
+    return bad
+
+
+def test_bad_columns_in_every_matrix_rejected():
+    A, F = _small()
+    b = cmamd.csr_mv_host(A, np.ones(A.n))
+    cases = [
+        lambda: cmamd.System("afiim", _bad_col(A, -1), F),
+        lambda: cmamd.System("afiim", A, cmamd.Factors(_bad_col(F.G, A.n), F.idv, F.H)),
+        lambda: cmamd.System("afiim", A, cmamd.Factors(F.G, F.idv, _bad_col(F.H, 10 * A.n))),
+        lambda: cmamd.afiim_epbicgstab(_bad_col(A, A.n + 3), b, F, nmax=50),
+    ]
+    Fc = cmamd.acifim(A, 0.01, 0.1)
+    for mode, nr in (("C", 2), ("I", 1), ("J", 1)):  # L checked on the device (C) or on the host (I, J)
+        Fb = cmamd.Factors(Fc.G, Fc.idv, Fc.H, _bad_col(Fc.L, A.n), Fc.U)
+        cases.append(lambda Fb=Fb, mode=mode, nr=nr: cmamd.System("acifim", A, Fb, nr=nr, mode=mode))
+    # symmetric PDFIIM: the stored triangle is expanded on the host
+    Fp = cmamd.pdfiim(A, 1, 1, 1, 0.05, -1, "I", "B")
+    cases.append(lambda: cmamd.System("pdfiim", _bad_col(A, A.n), Fp, a_symmetric=True))
+    for make in cases:
+        with pytest.raises(cmamd.EngineError) as e:
+            make()
+        assert e.value.code == -2 and "out of range" in str(e.value)
+    s = cmamd.System("afiim", A, F)
+    s.set_rhs(b)
     s.set_x(None)
     assert s.solve(tol=1e-8, nmax=200)[2] == 0
     s.close()

@@ -742,6 +742,11 @@ __global__ __launch_bounds__(BS) void spmv_dia(int n, int ldn, int nd, const int
         Acc s[RPT];
 #pragma unroll
         for (int r = 0; r < RPT; r++) s[r] = 0;
+        // the rows' remainder bounds, loaded ahead of the diagonals so their
+        // latency hides behind the diagonal loads
+        int rb[RPT + 1];
+#pragma unroll
+        for (int r = 0; r <= RPT; r++) rb[r] = rptr ? rptr[min(i0 + r, n)] : 0;
         using VT = typename DiaVec<V, RPT>::type;
         for (int d0 = 0; d0 < nd; d0 += CHUNK) {
             VT v[CHUNK];
@@ -780,8 +785,7 @@ __global__ __launch_bounds__(BS) void spmv_dia(int n, int ldn, int nd, const int
             const int i = i0 + r;
             if (i >= n) break;
             Acc t = s[r];
-            if (rptr)
-                for (int k = rptr[i]; k < rptr[i + 1]; k++) t += load_val<V, Acc>(rval, k) * (Acc)x[rcol[k]];
+            for (int k = rb[r]; k < rb[r + 1]; k++) t += load_val<V, Acc>(rval, k) * (Acc)x[rcol[k]];
             if (rowscale) t *= (Acc)rowscale[i];
             epi.row(i, t, acc);
         }

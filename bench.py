@@ -31,6 +31,7 @@ import numpy as np  # noqa: E402
 HBM_PEAK_GBS = 8000.0
 SAMPLE_PASSES = 8  # passes per timed solve whose dominant-kernel launches are bracketed by HIP events
 DROPIN_CALLS = 3  # one-shot drop-in calls timed per single-system workload
+DOM_LAUNCHES_PER_PASS_MAX = 16  # above this, the dominant kernel is timed outside the timed region
 METRIC = "PBiCGSTAB solves/s + p99 latency at rel-res 1e-8; SpMV/M-apply HBM GB/s vs peak"
 
 from cmamd.workloads import WORKLOADS, build as build_workload, serve_requests, serve_systems  # noqa: E402
@@ -798,8 +799,14 @@ def bench_single(args, w):
     dom = max((c for c in ("A", "F1", "F2", "LU") if stats[c][0] > 0), key=lambda c: stats[c][1])
     # timed region: only the dominant kernel's launches carry events, and only in the first
     # SAMPLE_PASSES passes of each solve (launched kernel by kernel); the other passes replay from
-    # the captured graphs, as in an unprofiled solve
-    sys_.profile(args.kernel_timing == "timed-region", cats=(dom,), sample=SAMPLE_PASSES)
+    # the captured graphs, as in an unprofiled solve.  A dominant category of many launches per pass
+    # (the level-scheduled triangular solves: thousands per pass) is timed in separate solves
+    # instead: events around each of its launches would slow the timed solves themselves.
+    profiled_passes = max(1, stats["A"][0] // 2)  # passes of the two profiled solves (two A products each)
+    kernel_timing = args.kernel_timing
+    if stats[dom][0] / profiled_passes > DOM_LAUNCHES_PER_PASS_MAX:
+        kernel_timing = "separate"
+    sys_.profile(kernel_timing == "timed-region", cats=(dom,), sample=SAMPLE_PASSES)
     lat = []
     iters = []
     t0 = time.perf_counter()
@@ -810,7 +817,7 @@ def bench_single(args, w):
         lat.append(time.perf_counter() - ts)
         iters.append(it)
     elapsed = time.perf_counter() - t0
-    if args.kernel_timing == "separate":
+    if kernel_timing == "separate":
         sys_.profile(True, cats=(dom,))
         for _ in range(3):
             sys_.set_x(None)
@@ -913,9 +920,12 @@ def bench_single(args, w):
                                    "host arrays, each uploading and checking A, the factors and b, solving and "
                                    "copying x back (not the bench value)"},
         "roofline": roofline, "kernels": kernels,
-        "kernels_note": "per-kernel table from 2 extra solves with every launch bracketed by HIP events; "
-                        "roofline from the timed region, where only the dominant kernel is bracketed, in the "
-                        f"first {SAMPLE_PASSES} passes of each solve (the rest replay from hipGraphs)",
+        "kernels_note": "per-kernel table from 2 extra solves with every launch bracketed by HIP events; " + (
+            "roofline from the timed region, where only the dominant kernel is bracketed, in the "
+            f"first {SAMPLE_PASSES} passes of each solve (the rest replay from hipGraphs)"
+            if kernel_timing == "timed-region" else
+            "roofline from 3 solves after the timed region with the dominant kernel's launches bracketed "
+            "(the timed solves carry no events)"),
     }
     if not args.no_cpu:
         out["cpu_baseline"], x_or = cpu_baseline(w, A, b, F, args.cpu_seconds)

@@ -183,6 +183,19 @@ static void check_levels(const Csr &A)
                 const int j = A.col[k];
                 if (lower ? j < i : j > i) CHECK(lev[j] < lev[i], "dependency in an earlier level");
             }
+        // the triangle's entries in level order: each row's own, in its order, nothing else
+        CHECK((int)L.tptr.size() == A.n + 1 && L.tptr[0] == 0 && L.tptr[A.n] == (int)L.tcol.size() &&
+                  L.tsrc.size() == L.tcol.size(),
+              "level-ordered triangle shape");
+        for (int q = 0; q < A.n; q++) {
+            const int i = L.rows[q];
+            std::vector<int> want;
+            for (int k = A.ptr[i]; k < A.ptr[i + 1]; k++)
+                if (lower ? A.col[k] <= i : A.col[k] >= i) want.push_back(k);
+            std::vector<int> got(L.tsrc.begin() + L.tptr[q], L.tsrc.begin() + L.tptr[q + 1]);
+            CHECK(got == want, "level-ordered row = the row's triangle entries");
+            for (int p2 = L.tptr[q]; p2 < L.tptr[q + 1]; p2++) CHECK(L.tcol[p2] == A.col[L.tsrc[p2]], "column kept");
+        }
     }
 }
 

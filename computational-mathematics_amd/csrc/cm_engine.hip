@@ -384,8 +384,13 @@ struct DevCsr {
     bool lower = false;  // the level schedule's triangle (lower: rows read smaller columns)
     int *lrows = nullptr;
     std::vector<int> loff;
-    std::vector<long long> lnnz;  // per level: nonzeros of its rows
+    std::vector<long long> lnnz;  // per level: the triangle's entries of its rows
     int lpr = 1;                  // lanes per row of the level kernel
+    // the triangle's entries in level order (tptr indexed by level position;
+    // tsrc: CSR positions; tval: this copy's values gathered through tsrc)
+    int *tptr = nullptr, *tcol = nullptr, *tsrc = nullptr;
+    long long ntri = 0;
+    V *tval = nullptr;
     bool present() const { return ptr != nullptr; }
     // device bytes held (index arrays only when owned)
     size_t bytes() const
@@ -394,7 +399,9 @@ struct DevCsr {
         size_t b = (size_t)nnz * sizeof(V) + (rowscale ? (size_t)n * sizeof(float) : 0);
         if (sval) b += (size_t)nnz * sizeof(V);
         if (dval && dia) b += ((size_t)dia->nd * dia->ldn + (size_t)dia->nrem + (size_t)dia->nlrem) * sizeof(V);
+        if (tval) b += (size_t)ntri * sizeof(V);
         if (owns_index) {
+            if (tptr) b += ((size_t)n + 1 + 2 * (size_t)ntri) * sizeof(int);
             if (slots) b += slots->bytes();
             if (dia) b += dia->bytes();
             b += (size_t)(n + 1 + 2 * (nblk + 1)) * sizeof(int) + (size_t)nnz * sizeof(int) + (lrows ? (size_t)n * sizeof(int) : 0);
@@ -411,7 +418,11 @@ struct DevCsr {
             (void)hipFree(lrows);
             (void)hipFree(d16);
             (void)hipFree(cbase);
+            (void)hipFree(tptr);
+            (void)hipFree(tcol);
+            (void)hipFree(tsrc);
         }
+        (void)hipFree(tval);
         (void)hipFree(val);
         (void)hipFree(rowscale);
         (void)hipFree(sval);
@@ -419,8 +430,10 @@ struct DevCsr {
         (void)hipFree(rval);
         (void)hipFree(lval);
         ptr = col = blk = lrows = cbase = nullptr;
+        tptr = tcol = tsrc = nullptr;
+        ntri = 0;
         d16 = nullptr;
-        val = sval = dval = rval = lval = nullptr;
+        val = sval = dval = rval = lval = tval = nullptr;
         rowscale = nullptr;
         slots.reset();
         dia.reset();
@@ -501,6 +514,10 @@ static int cast_csr(DevCsr<V> &dst, const DevCsr<double> &src, hipStream_t s)
     dst.loff = src.loff;
     dst.lnnz = src.lnnz;
     dst.lpr = src.lpr;
+    dst.tptr = src.tptr;
+    dst.tcol = src.tcol;
+    dst.tsrc = src.tsrc;
+    dst.ntri = src.ntri;
     CM_TRY(dev_alloc(&dst.val, (size_t)dst.nnz + CSR_PAD));
     CM_HIP(hipMemsetAsync(dst.val + dst.nnz, 0, CSR_PAD * sizeof(V), s));
     if constexpr (std::is_same<V, __half>::value) {
@@ -508,6 +525,10 @@ static int cast_csr(DevCsr<V> &dst, const DevCsr<double> &src, hipStream_t s)
         k_half_rows<<<(dst.n + BS - 1) / BS, BS, 0, s>>>(dst.n, src.ptr, src.val, dst.val, dst.rowscale);
     } else {
         k_cast<V><<<long_grid(dst.nnz), BS, 0, s>>>(dst.nnz, src.val, dst.val);
+    }
+    if (dst.tsrc && dst.ntri) {  // the triangle's values in level order, from this copy's values
+        CM_TRY(dev_alloc(&dst.tval, (size_t)dst.ntri));
+        k_gather_vals<V><<<long_grid(dst.ntri), BS, 0, s>>>(dst.ntri, dst.tsrc, dst.val, dst.tval);
     }
     CM_HIP(hipGetLastError());
     return CM_OK;
@@ -877,23 +898,79 @@ static int build_dia_vals(DevCsr<V> &m, hipStream_t s)
     return CM_OK;
 }
 
-// Dependency levels of a triangular factor (level_schedule): one launch of
-// the triangular-solve kernel per level.
+// Per level position q (row rows[q]): the count of its row's entries in the
+// declared triangle (lower: j <= i, upper: j >= i).
+__global__ __launch_bounds__(BS) void k_tri_count(int n, const int *__restrict__ rows, const int *__restrict__ ptr,
+                                                  const int *__restrict__ col, bool lower, int *cnt)
+{
+    for (int q = blockIdx.x * BS + threadIdx.x; q < n; q += gridDim.x * BS) {
+        const int i = rows[q];
+        int c = 0;
+        for (int k = ptr[i]; k < ptr[i + 1]; k++) c += lower ? col[k] <= i : col[k] >= i;
+        cnt[q] = c;
+    }
+}
+
+// The triangle's entries in level order: columns and CSR positions from tptr[q].
+__global__ __launch_bounds__(BS) void k_tri_fill(int n, const int *__restrict__ rows, const int *__restrict__ ptr,
+                                                 const int *__restrict__ col, bool lower,
+                                                 const int *__restrict__ tptr, int *tcol, int *tsrc)
+{
+    for (int q = blockIdx.x * BS + threadIdx.x; q < n; q += gridDim.x * BS) {
+        const int i = rows[q];
+        int pos = tptr[q];
+        for (int k = ptr[i]; k < ptr[i + 1]; k++) {
+            const int j = col[k];
+            if (lower ? j <= i : j >= i) {
+                tcol[pos] = j;
+                tsrc[pos] = k;
+                pos++;
+            }
+        }
+    }
+}
+
+// Dependency levels of a triangular factor (level_schedule, on the host):
+// one launch of the triangular-solve kernel per level; the triangle's
+// entries are then gathered in level order on the device.
 static int build_levels(DevCsr<double> &m, const int *ptr, const int *col, bool lower, hipStream_t s)
 {
-    LevelSchedule L = level_schedule(m.n, ptr, col, lower);
+    LevelSchedule L = level_schedule(m.n, ptr, col, lower, false);
     m.nlev = L.nlev;
     m.lower = lower;
     m.loff = L.loff;
-    m.lnnz.assign(L.nlev, 0);  // entries each level's rows hold (read by its launch)
-    for (int l = 0; l < L.nlev; l++)
-        for (int q = L.loff[l]; q < L.loff[l + 1]; q++) m.lnnz[l] += ptr[L.rows[q] + 1] - ptr[L.rows[q]];
-    // lanes per row: the power of two at or above the mean stored row length, 1..16
-    const double mean = m.n ? (double)ptr[m.n] / m.n : 0.0;
+    CM_TRY(dev_upload(&m.lrows, L.rows.data(), (size_t)m.n, s));
+    // the triangle's entries in level order
+    const int n = m.n;
+    DevTemps tmp;
+    int *cnt;
+    CM_TRY(tmp.alloc(&cnt, (size_t)n));
+    CM_TRY(dev_alloc(&m.tptr, (size_t)n + 1));
+    CM_HIP(hipMemsetAsync(m.tptr, 0, sizeof(int), s));
+    const int g = std::min((n + BS - 1) / BS, 8192);
+    k_tri_count<<<g, BS, 0, s>>>(n, m.lrows, m.ptr, m.col, lower, cnt);
+    size_t tb = 0;
+    CM_HIP(hipcub::DeviceScan::InclusiveSum(nullptr, tb, cnt, m.tptr + 1, n, s));
+    char *tbuf;
+    CM_TRY(tmp.alloc(&tbuf, std::max<size_t>(tb, 1)));
+    CM_HIP(hipcub::DeviceScan::InclusiveSum((void *)tbuf, tb, cnt, m.tptr + 1, n, s));
+    std::vector<int> tp((size_t)n + 1);
+    CM_HIP(hipMemcpyAsync(tp.data(), m.tptr, tp.size() * sizeof(int), hipMemcpyDeviceToHost, s));
+    CM_HIP(hipStreamSynchronize(s));
+    m.ntri = tp[n];
+    m.lnnz.assign(L.nlev, 0);  // the triangle's entries each level's rows hold (read by its launch)
+    for (int l = 0; l < L.nlev; l++) m.lnnz[l] = tp[L.loff[l + 1]] - tp[L.loff[l]];
+    CM_TRY(dev_alloc(&m.tcol, (size_t)m.ntri));
+    CM_TRY(dev_alloc(&m.tsrc, (size_t)m.ntri));
+    CM_TRY(dev_alloc(&m.tval, (size_t)m.ntri));
+    k_tri_fill<<<g, BS, 0, s>>>(n, m.lrows, m.ptr, m.col, lower, m.tptr, m.tcol, m.tsrc);
+    if (m.ntri) k_gather_vals<double><<<long_grid(m.ntri), BS, 0, s>>>(m.ntri, m.tsrc, m.val, m.tval);
+    // lanes per row: the power of two at or above the triangle's mean row length, 1..16
+    const double mean = m.n ? (double)m.ntri / m.n : 0.0;
     m.lpr = 1;
     while (m.lpr < 16 && m.lpr < mean) m.lpr *= 2;
-    CM_TRY(dev_upload(&m.lrows, L.rows.data(), (size_t)m.n, s));
-    CM_HIP(hipStreamSynchronize(s));
+    CM_HIP(hipGetLastError());
+    CM_HIP(hipStreamSynchronize(s));  // before the temporaries are freed
     return CM_OK;
 }
 
@@ -1245,12 +1322,12 @@ struct Krylov {
             const double by = (double)m.lnnz[l] * (sizeof(VF) + 4.0) + (double)cnt * (4.0 + 8.0 + 3.0 * sizeof(T));
             timed(KC_LU, by, [&] {
                 const SolveState *gate = use_state ? st : nullptr;
-                const int *lr = m.lrows + m.loff[l];
+                const int q0 = m.loff[l];
                 auto go = [&](auto lanes) {
                     constexpr int LPR = decltype(lanes)::value;
                     const int g = (int)(((long long)cnt * LPR + BS - 1) / BS);
-                    k_trsv_level<T, VF, LPR><<<g, BS, 0, stream>>>(cnt, m.lower, lr, m.ptr, m.col, m.val, m.rowscale,
-                                                                   bin, xout, gate);
+                    k_trsv_level<T, VF, LPR><<<g, BS, 0, stream>>>(cnt, m.lrows + q0, m.tptr + q0, m.tcol, m.tval,
+                                                                   m.rowscale, bin, xout, gate);
                 };
                 switch (m.lpr) {
                 case 1: go(std::integral_constant<int, 1>{}); break;

@@ -164,12 +164,18 @@ inline void host_sym_expand(int n, const int *ptr, const int *col, const double 
 // smaller columns (rows in increasing order); upper -> larger columns.
 // Entries on the other side of the diagonal are ignored.  rows lists the
 // rows level by level (loff[l] .. loff[l + 1]).
+// Dependency levels of a triangular factor: rows[loff[l] .. loff[l + 1]) are
+// level l's rows.  The declared triangle's entries (diagonal included) are
+// also stored in that row order, tptr / tcol / tsrc (tsrc: the entry's CSR
+// position), so a level's rows read their entries from one contiguous run.
 struct LevelSchedule {
     int nlev = 0;
-    std::vector<int> loff, rows;
+    std::vector<int> loff, rows, tptr, tcol, tsrc;
 };
 
-inline LevelSchedule level_schedule(int n, const int *ptr, const int *col, bool lower)
+// (with_triangle false: loff and rows only; the engine gathers the
+// triangle's entries on the device)
+inline LevelSchedule level_schedule(int n, const int *ptr, const int *col, bool lower, bool with_triangle = true)
 {
     LevelSchedule L;
     std::vector<int> lev(n, 0);
@@ -189,6 +195,21 @@ inline LevelSchedule level_schedule(int n, const int *ptr, const int *col, bool 
     L.rows.assign(n, 0);
     std::vector<int> fill(L.loff.begin(), L.loff.end() - 1);
     for (int i = 0; i < n; i++) L.rows[fill[lev[i]]++] = i;
+    if (!with_triangle) return L;
+    L.tptr.assign(n + 1, 0);
+    L.tcol.clear();
+    L.tsrc.clear();
+    for (int q = 0; q < n; q++) {
+        const int i = L.rows[q];
+        for (int k = ptr[i]; k < ptr[i + 1]; k++) {
+            const int j = col[k];
+            if (lower ? j <= i : j >= i) {
+                L.tcol.push_back(j);
+                L.tsrc.push_back(k);
+            }
+        }
+        L.tptr[q + 1] = (int)L.tcol.size();
+    }
     return L;
 }
 

@@ -3819,7 +3819,8 @@ int cm_system_kernel_format_bytes(cm_system h, int category, double *total_bytes
 // (built on the device) for each SpMV matrix whose diagonals held by 60% of
 // the rows leave at most ONESHOT_DIA_REM of its nonzeros to the remainder --
 // the second variant from ONESHOT_DIA2_ND diagonals up, where it times
-// faster -- the staged row blocks otherwise.
+// faster -- the staged row blocks otherwise, also for a matrix the device
+// build leaves to the host (whose index would take longer than the solve).
 constexpr double ONESHOT_DIA_REM = 0.3;
 constexpr int ONESHOT_DIA2_ND = 12;
 static int oneshot_formats(System &S)
@@ -3827,7 +3828,8 @@ static int oneshot_formats(System &S)
     K64 &k = *S.k64;
     for (DevCsr<double> *m : {&k.A, &k.G, &k.H, &k.GT}) {
         if (!m->present()) continue;
-        CM_TRY(build_dia(*m, k.stream));
+        bool done = false;
+        if (!m->dia) CM_TRY(build_dia_device(*m, k.stream, &done));
         if (!m->dia || (double)m->dia->nrem > ONESHOT_DIA_REM * (double)m->nnz) continue;
         CM_TRY(set_format(*m, m->dia->nd >= ONESHOT_DIA2_ND ? FMT_DIA2 : FMT_DIA, k.stream));
         CM_TRY(k.ensure_partials((m->n + BS - 1) / BS));

@@ -235,16 +235,6 @@ static void check_dia(const Csr &A)
     if (!dia_layout(A.n, A.ptr.data(), A.col.data(), D)) return;
     CHECK(D.nd >= 1 && D.nd <= DIA_MAX && D.ldn >= A.n && D.ldn % DIA_ALIGN == 0, "diagonal shape");
     for (int d = 1; d < D.nd; d++) CHECK(D.off[d] > D.off[d - 1], "offsets ascending");
-    // runs: the diagonals in order, each run consecutive offsets, maximal up to DIA_RUN_MAX
-    int next = 0;
-    for (size_t j = 0; j < D.runs.size(); j++) {
-        const int d0 = D.runs[j] & 0xff, m = D.runs[j] >> 8;
-        CHECK(d0 == next && m >= 1 && m <= DIA_RUN_MAX && d0 + m <= D.nd, "run covers the next diagonals");
-        for (int q = 1; q < m && d0 + q < D.nd; q++) CHECK(D.off[d0 + q] == D.off[d0] + q, "run offsets consecutive");
-        if (m < DIA_RUN_MAX && d0 + m < D.nd) CHECK(D.off[d0 + m] != D.off[d0] + m, "run is maximal");
-        next = d0 + m;
-    }
-    CHECK(next == D.nd, "runs cover every diagonal");
     std::vector<int> seen(A.col.size(), 0);
     for (int d = 0; d < D.nd; d++)
         for (int i = 0; i < D.ldn; i++) {

@@ -261,151 +261,83 @@ __global__ __launch_bounds__(BS) void spmm_short(int cnt, const int *__restrict_
     block_colgroup_partials<Epi::K, KB, CPL>(acc, red, partials, slot0 + blockIdx.x);
 }
 
-// Value of lane J of each aligned group of G lanes (G = 4 or 8), in every
-// lane of the group: a DPP quad permute for G = 4, a bit-mode swizzle (lane
-// (id & ~(G - 1)) | J of each 32) for G = 8.  64-bit values move as two
-// 32-bit halves.
-template <int G, int J>
-__device__ __forceinline__ int grp_lane_i(int a)
-{
-    static_assert((G == 4 || G == 8) && J >= 0 && J < G, "groups of 4 or 8 lanes");
-    if constexpr (G == 4)
-        return __builtin_amdgcn_mov_dpp(a, J * 0x55, 0xf, 0xf, false);
-    else
-        return __builtin_amdgcn_ds_swizzle(a, (0x1f & ~(G - 1)) | (J << 5));
-}
-template <int G, int J>
-__device__ __forceinline__ float grp_lane(float a)
-{
-    return __int_as_float(grp_lane_i<G, J>(__float_as_int(a)));
-}
-template <int G, int J>
-__device__ __forceinline__ double grp_lane(double a)
-{
-    const long long b = __double_as_longlong(a);
-    const int lo = grp_lane_i<G, J>((int)(b & 0xffffffffLL)), hi = grp_lane_i<G, J>((int)(b >> 32));
-    return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
-}
-// v[j] = lane j's a, j < R (groups of G >= R lanes)
-template <int G, int R, int J = 0>
-struct GrpSpread {
-    template <class A>
-    __device__ static __forceinline__ void go(A a, A (&v)[R])
-    {
-        if constexpr (J < R) {
-            v[J] = grp_lane<G, J>(a);
-            GrpSpread<G, R, J + 1>::go(a, v);
-        }
-    }
-};
-
 // Diagonals plus a CSR remainder (the single-RHS spmv_dia layout: dval[d *
 // ldn + i] = A(i, i + off[d]), remainder rptr / rcol / rval, rptr null when
-// empty) over KB columns.  Each group of LPRW lanes (CPL columns per lane)
-// takes R = DIA_RUN_ROWS consecutive rows r0 .. r0 + R - 1, rows walked in
-// contiguous chunks per workgroup.  The diagonals come in runs of
-// consecutive offsets o .. o + m - 1 (runs[j] = first diagonal | m << 8,
-// m <= DIA_RUN_MAX): the R rows' x segments for a run are the R + m - 1
-// consecutive rows from r0 + o, each loaded once and used by up to m
-// products, where one load per (row, diagonal) would be R m.  A diagonal's
-// values for the R rows are one coalesced load per lane (row r0 + lane % R
-// of the group) spread over the group by lane exchanges when LPRW >= R, R
-// loads per lane otherwise; no column index is read for the diagonal part.
-// Per row the sums run in diagonal order, then the remainder.  Rows with
-// long remainders (lpos[row] >= 0; lpos null when there are none) take their
-// remainder sums from racc (KB values per long row, written before this
-// launch from the remainders' column-tiled segments) instead of walking the
-// remainder themselves.  dval rows up to ldn >= n rounded up to R are read.
+// empty) over KB columns: LPRW lanes per row, each with CPL columns, rows
+// walked in contiguous chunks per workgroup like spmm_short.  A diagonal's
+// value is one load per row (shared by its lanes), its x segment the row's
+// KB columns at row i + off[d], so consecutive rows read consecutive x
+// segments and no column index is read for the diagonal part.  Row sums in
+// diagonal order, then the remainder.  Rows with long remainders (lpos[row]
+// >= 0; lpos null when there are none) take their remainder sums from racc
+// (KB values per long row, written before this launch from the remainders'
+// column-tiled segments) instead of walking the remainder themselves.
 template <class V, class T, class Acc, int KB, class Epi>
-__global__ __launch_bounds__(BS) void spmm_dia(int n, int ldn, int nrun, const int *__restrict__ runs,
-                                               const int *__restrict__ off, const V *__restrict__ dval,
-                                               const int *__restrict__ rptr, const int *__restrict__ rcol,
-                                               const V *__restrict__ rval, const int *__restrict__ lpos,
-                                               const T *__restrict__ racc, const float *__restrict__ rowscale,
-                                               const T *__restrict__ x, Epi epi, const BlockFlags *fl,
-                                               double *partials, int slot0)
+__global__ __launch_bounds__(BS) void spmm_dia(int n, int ldn, int nd, const int *__restrict__ off,
+                                               const V *__restrict__ dval, const int *__restrict__ rptr,
+                                               const int *__restrict__ rcol, const V *__restrict__ rval,
+                                               const int *__restrict__ lpos, const T *__restrict__ racc,
+                                               const float *__restrict__ rowscale, const T *__restrict__ x, Epi epi,
+                                               const BlockFlags *fl, double *partials, int slot0)
 {
     if (fl && fl->all_done) return;
     constexpr int CPL = ColSplit<T, KB>::CPL, LPRW = ColSplit<T, KB>::LPRW, KK = Epi::K > 0 ? Epi::K : 1;
-    constexpr int R = DIA_RUN_ROWS, MR = DIA_RUN_MAX, NX = R + MR - 1;
-    static_assert(DIA_ALIGN % R == 0, "diagonal stride must cover whole row groups");
+    constexpr int CH = 4;  // diagonals in flight per step
     __shared__ double red[(BS / WAVE) * KK * KB];
     double acc[CPL][KK];
 #pragma unroll
     for (int i = 0; i < CPL; i++)
 #pragma unroll
         for (int k = 0; k < KK; k++) acc[i][k] = 0.0;
-    const int lg = threadIdx.x % LPRW, c0 = lg * CPL;
-    constexpr int STEP = BS / LPRW * R;  // rows per workgroup step
-    const int chunk = ((n + (int)gridDim.x - 1) / (int)gridDim.x + STEP - 1) / STEP * STEP;
+    const int c0 = (threadIdx.x % LPRW) * CPL;
+    constexpr int SR = BS / LPRW;
+    const int chunk = ((n + (int)gridDim.x - 1) / (int)gridDim.x + SR - 1) / SR * SR;
     const int r_begin = xcd_block(blockIdx.x, gridDim.x) * chunk, r_end = min(n, r_begin + chunk);
-    for (int r0 = r_begin + (int)threadIdx.x / LPRW * R; r0 < r_end; r0 += STEP) {
+    for (int r = r_begin + (int)threadIdx.x / LPRW; r < r_end; r += SR) {
 #pragma clang fp contract(off)
-        Acc s[R][CPL];
+        Acc s[CPL];
 #pragma unroll
-        for (int q = 0; q < R; q++)
+        for (int i = 0; i < CPL; i++) s[i] = 0;
+        for (int d0 = 0; d0 < nd; d0 += CH) {
+            Acc v[CH];
+            T xv[CH][CPL];
 #pragma unroll
-            for (int i = 0; i < CPL; i++) s[q][i] = 0;
-        for (int j = 0; j < nrun; j++) {
-            const int rj = runs[j], d0 = rj & 0xff, m = rj >> 8, o = off[d0];
-            Acc v[MR][R];
+            for (int q = 0; q < CH; q++) {
+                const int d = d0 + q;
+                v[q] = 0;
 #pragma unroll
-            for (int q = 0; q < MR; q++)
-                if (q < m) {
-                    const V *dv = dval + (size_t)(d0 + q) * ldn + r0;
-                    if constexpr (LPRW >= R) {
-                        GrpSpread<LPRW, R>::go(load_val<V, Acc>(dv, lg % R), v[q]);
-                    } else {
-#pragma unroll
-                        for (int t = 0; t < R; t++) v[q][t] = load_val<V, Acc>(dv, t);
-                    }
+                for (int i = 0; i < CPL; i++) xv[q][i] = 0;
+                if (d < nd) {
+                    const int c = r + off[d];
+                    v[q] = load_val<V, Acc>(dval + (size_t)d * ldn, r);
+                    if ((unsigned)c < (unsigned)n) ld_cols<T, CPL>(x + (size_t)c * KB + c0, xv[q]);
                 }
-            T xv[NX][CPL];
-#pragma unroll
-            for (int t = 0; t < NX; t++) {
-#pragma unroll
-                for (int i = 0; i < CPL; i++) xv[t][i] = 0;
-                const int c = r0 + o + t;
-                if (t < R + m - 1 && (unsigned)c < (unsigned)n) ld_cols<T, CPL>(x + (size_t)c * KB + c0, xv[t]);
             }
 #pragma unroll
-            for (int q = 0; q < MR; q++)
-                if (q < m) {
+            for (int q = 0; q < CH; q++)
+                if (d0 + q < nd) {
 #pragma unroll
-                    for (int t = 0; t < R; t++)
-#pragma unroll
-                        for (int i = 0; i < CPL; i++) s[t][i] += v[q][t] * (Acc)xv[t + q][i];
+                    for (int i = 0; i < CPL; i++) s[i] += v[q] * (Acc)xv[q][i];
                 }
         }
-        // the R rows' long-row positions and remainder bounds, loaded together
-        int lq[R], rb[R + 1];
+        const int lq = lpos ? lpos[r] : -1;
+        if (lq >= 0) {
+            T xr[CPL];
+            ld_cols<T, CPL>(racc + (size_t)lq * KB + c0, xr);
 #pragma unroll
-        for (int t = 0; t < R; t++) lq[t] = lpos && r0 + t < r_end ? lpos[r0 + t] : -1;
-#pragma unroll
-        for (int t = 0; t <= R; t++) rb[t] = rptr ? rptr[min(r0 + t, n)] : 0;
-#pragma unroll
-        for (int t = 0; t < R; t++) {
-            const int r = r0 + t;
-            if (r >= r_end) break;
-            if (lq[t] >= 0) {
+            for (int i = 0; i < CPL; i++) s[i] += (Acc)xr[i];
+        } else if (rptr) {
+            for (int k = rptr[r]; k < rptr[r + 1]; k++) {
                 T xr[CPL];
-                ld_cols<T, CPL>(racc + (size_t)lq[t] * KB + c0, xr);
+                ld_cols<T, CPL>(x + (size_t)rcol[k] * KB + c0, xr);
+                const Acc vk = load_val<V, Acc>(rval, k);
 #pragma unroll
-                for (int i = 0; i < CPL; i++) s[t][i] += (Acc)xr[i];
-            } else {
-                for (int k = rb[t]; k < rb[t + 1]; k++) {
-                    T xr[CPL];
-                    ld_cols<T, CPL>(x + (size_t)rcol[k] * KB + c0, xr);
-                    const Acc vk = load_val<V, Acc>(rval, k);
-#pragma unroll
-                    for (int i = 0; i < CPL; i++) s[t][i] += vk * (Acc)xr[i];
-                }
+                for (int i = 0; i < CPL; i++) s[i] += vk * (Acc)xr[i];
             }
-            const Acc sc = rowscale ? (Acc)rowscale[r] : (Acc)1;
-#pragma unroll
-            for (int i = 0; i < CPL; i++)
-                epi.row((size_t)r * KB + c0 + i, c0 + i, rowscale ? s[t][i] * sc : s[t][i], acc[i]);
         }
+        const Acc sc = rowscale ? (Acc)rowscale[r] : (Acc)1;
+#pragma unroll
+        for (int i = 0; i < CPL; i++) epi.row((size_t)r * KB + c0 + i, c0 + i, rowscale ? s[i] * sc : s[i], acc[i]);
     }
     block_colgroup_partials<Epi::K, KB, CPL>(acc, red, partials, slot0 + blockIdx.x);
 }

@@ -301,9 +301,8 @@ struct SlotIndex {
 // the entries off those diagonals with their CSR positions.
 struct DiaIndex {
     int n = 0, ldn = 0, nd = 0;  // ldn: diagonal stride, n rounded up to DIA_ALIGN
-    int nrun = 0;                // runs: consecutive-offset runs of the diagonals (first | length << 8)
     long long nrem = 0;
-    int *off = nullptr, *runs = nullptr, *dsrc = nullptr, *rptr = nullptr, *rcol = nullptr, *rsrc = nullptr;
+    int *off = nullptr, *dsrc = nullptr, *rptr = nullptr, *rcol = nullptr, *rsrc = nullptr;
     // Rows whose remainder is longer than SHORT_MAX, for the block products:
     // lpos[row] = their index q (0..nlong-1), -1 for the other rows.  Their
     // remainders are cut into ntile column tiles of tw columns and stored as a
@@ -318,14 +317,13 @@ struct DiaIndex {
     int *lpos = nullptr, *lptr = nullptr, *lcol = nullptr, *lsrc = nullptr, *lrows = nullptr;
     size_t bytes() const
     {
-        return ((size_t)nd + (size_t)nrun + (size_t)nd * ldn + (rptr ? (size_t)n + 1 : 0) + 2 * (size_t)nrem +
+        return ((size_t)nd + (size_t)nd * ldn + (rptr ? (size_t)n + 1 : 0) + 2 * (size_t)nrem +
                 (nlong ? (size_t)n + 2 * (size_t)nseg + 1 + 2 * (size_t)nlrem : 0)) *
                sizeof(int);
     }
     ~DiaIndex()
     {
         (void)hipFree(off);
-        (void)hipFree(runs);
         (void)hipFree(dsrc);
         (void)hipFree(rptr);
         (void)hipFree(rcol);
@@ -774,15 +772,12 @@ static int build_dia_device(DevCsr<double> &m, hipStream_t s, bool *done)
     const std::vector<int> off = dia_select(n, counted);
     *done = true;
     if (off.empty()) return CM_OK;
-    const std::vector<int> runs = dia_runs(off);
     const int nd = (int)off.size(), ldn = (n + DIA_ALIGN - 1) / DIA_ALIGN * DIA_ALIGN;
     auto D = std::make_shared<DiaIndex>();
     D->n = n;
     D->ldn = ldn;
     D->nd = nd;
-    D->nrun = (int)runs.size();
     CM_TRY(dev_upload(&D->off, off.data(), off.size(), s));
-    CM_TRY(dev_upload(&D->runs, runs.data(), runs.size(), s));
     // positions on the selected diagonals (map: offset -> diagonal), remainder counts
     k_fill_int<<<long_grid((long long)nmap), BS, 0, s>>>((long long)nmap, -1, map);
     k_dia_mark<<<(nd + BS - 1) / BS, BS, 0, s>>>(nd, D->off, n, map);
@@ -849,7 +844,6 @@ static int build_dia(DevCsr<double> &m, hipStream_t s)
     D->n = n;
     D->ldn = L.ldn;
     D->nd = L.nd;
-    D->nrun = (int)L.runs.size();
     D->nrem = (long long)L.rcol.size();
     D->nlong = L.nlong;
     D->ntile = L.ntile;
@@ -867,7 +861,6 @@ static int build_dia(DevCsr<double> &m, hipStream_t s)
         CM_HIP(hipStreamSynchronize(s));  // host temporaries
     }
     CM_TRY(dev_upload(&D->off, L.off.data(), L.off.size(), s));
-    CM_TRY(dev_upload(&D->runs, L.runs.data(), L.runs.size(), s));
     CM_TRY(dev_upload(&D->dsrc, L.dsrc.data(), L.dsrc.size(), s));
     if (D->nrem) {
         CM_TRY(dev_upload(&D->rptr, L.rptr.data(), L.rptr.size(), s));
@@ -1899,11 +1892,7 @@ struct BlockKrylov {
     static constexpr int FIN_ONE = 256, FIN_PER = 64;
     static int short_grid(const RowBins &bn) { return std::min((bn.ns + SROWS - 1) / SROWS, SPMM_CAP); }
     static int long_grid(const RowBins &bn) { return std::min((bn.nl + (BS / WAVE) - 1) / (BS / WAVE), SPMM_CAP); }
-    // spmm_dia: DIA_RUN_ROWS rows per lane group
-    static int dia_run_grid(int nrows)
-    {
-        return std::min((nrows + SROWS * DIA_RUN_ROWS - 1) / (SROWS * DIA_RUN_ROWS), SPMM_CAP);
-    }
+    static int dia_grid(int nrows) { return std::min((nrows + SROWS - 1) / SROWS, SPMM_CAP); }
 
     // Elements of the long-row remainder sums one product of m needs: the
     // row sums, then the segment sums when its remainders are tiled.
@@ -1967,7 +1956,7 @@ struct BlockKrylov {
         CM_HIP(hipHostMalloc((void **)&hst, 3 * KB * sizeof(SolveState), hipHostMallocDefault));
         CM_HIP(hipMalloc((void **)&fl, sizeof(BlockFlags)));
         CM_HIP(hipHostMalloc((void **)&hfl, 3 * sizeof(BlockFlags), hipHostMallocDefault));
-        size_t slots = std::max((size_t)vgrid(), (size_t)dia_run_grid(n));
+        size_t slots = std::max((size_t)vgrid(), (size_t)dia_grid(n));
         for (const RowBins *bn : {&bA, &bG, &bH})
             slots = std::max(slots, (size_t)(short_grid(*bn) + long_grid(*bn)));
         partial_slots = std::max(slots, (size_t)MAX_RED_BLOCKS);
@@ -1988,7 +1977,7 @@ struct BlockKrylov {
         const BlockFlags *gf = gate ? fl : nullptr;
         if (m.block_dia && m.dval) {
             const DiaIndex &D = *m.dia;
-            const int gd = dia_run_grid(n);
+            const int gd = dia_grid(n);
             if (racc_of(&m) > racc_cap) {  // never launch through a missing or short buffer
                 racc_short = true;
                 return;
@@ -2005,9 +1994,9 @@ struct BlockKrylov {
                     k_long_tiles<T, Acc><<<gt, BS, 0, stream>>>(ne_l, D.ntile, seg, racc, gf);
                 }
             }
-            spmm_dia<V, T, Acc, KB, Epi><<<gd, BS, 0, stream>>>(n, D.ldn, D.nrun, D.runs, D.off, m.dval, D.rptr,
-                                                               D.rcol, m.rval, D.nlong ? D.lpos : nullptr, racc,
-                                                               m.rowscale, xin, epi, gf, partials, 0);
+            spmm_dia<V, T, Acc, KB, Epi><<<gd, BS, 0, stream>>>(n, D.ldn, D.nd, D.off, m.dval, D.rptr, D.rcol, m.rval,
+                                                               D.nlong ? D.lpos : nullptr, racc, m.rowscale, xin,
+                                                               epi, gf, partials, 0);
             if constexpr (Epi::K > 0) finalize<Epi::K>(gd, epi, fst, gate, gate ? 1 : 0);
             return;
         }
@@ -2097,7 +2086,7 @@ struct BlockKrylov {
 
     int fit_partials()
     {
-        size_t slots = std::max((size_t)vgrid(), (size_t)dia_run_grid(n));
+        size_t slots = std::max((size_t)vgrid(), (size_t)dia_grid(n));
         for (const RowBins *bn : {&bA, &bG, &bH})
             slots = std::max(slots, (size_t)(short_grid(*bn) + long_grid(*bn)));
         if (slots <= partial_slots) return CM_OK;

@@ -24,10 +24,6 @@ constexpr int DIA_MAX = 32;       // diagonals stored densely, at most
 constexpr int DIA_ALIGN = 8;      // diagonal stride: n rounded up to this
 constexpr double DIA_FILL = 0.6;  // a diagonal is stored densely when at least this share of rows has it
 constexpr int SHORT_MAX = 64;     // block products: longer rows go to the wave-per-row bin
-// Diagonal block products: rows taken DIA_RUN_ROWS at a time per lane group,
-// diagonals in runs of at most DIA_RUN_MAX consecutive offsets.
-constexpr int DIA_RUN_ROWS = 4, DIA_RUN_MAX = 4;
-static_assert(DIA_MAX < 256, "a run stores its first diagonal in 8 bits");
 // Column tiles of the long remainders of the diagonal block products: about
 // one tile per LONG_TILE_COLS columns (2 MB of x rows at 16 FP32 columns),
 // at most LONG_TILES_MAX, and at least LONG_SEG_MIN remainder entries per
@@ -292,12 +288,11 @@ inline bool slot_layout(int n, const int *ptr, const int *col, SlotLayout &S)
 // their remainders cut into ntile column tiles of tw columns, stored as a
 // tile-major CSR of (tile, row) segments (lptr, lcol, lsrc; segment t * nlong
 // + q holds long row q's entries with columns in [t tw, (t + 1) tw), in row
-// order).  runs: the diagonals as runs of consecutive offsets, at most
-// DIA_RUN_MAX long, each stored as its first diagonal | its length << 8.
+// order).
 struct DiaLayout {
     int ldn = 0, nd = 0, nlong = 0, ntile = 1, tw = 0;
     long long nseg = 0;
-    std::vector<int> off, runs, dsrc, rptr, rcol, rsrc, lpos, lptr, lcol, lsrc;
+    std::vector<int> off, dsrc, rptr, rcol, rsrc, lpos, lptr, lcol, lsrc;
 };
 
 // The stored diagonals from (entry count, offset) pairs: those held by at
@@ -318,21 +313,6 @@ inline std::vector<int> dia_select(int n, std::vector<std::pair<long long, int>>
     return off;
 }
 
-// Runs of consecutive offsets (ascending off), at most DIA_RUN_MAX long:
-// first diagonal | length << 8.
-inline std::vector<int> dia_runs(const std::vector<int> &off)
-{
-    std::vector<int> runs;
-    const int nd = (int)off.size();
-    for (int d = 0; d < nd;) {
-        int m = 1;
-        while (d + m < nd && m < DIA_RUN_MAX && off[d + m] == off[d] + m) m++;
-        runs.push_back(d | (m << 8));
-        d += m;
-    }
-    return runs;
-}
-
 // false when no diagonal qualifies (the format does not apply)
 inline bool dia_layout(int n, const int *ptr, const int *col, DiaLayout &D)
 {
@@ -347,7 +327,6 @@ inline bool dia_layout(int n, const int *ptr, const int *col, DiaLayout &D)
     D.off = dia_select(n, std::move(cand));
     if (D.off.empty()) return false;
     D.nd = (int)D.off.size();
-    D.runs = dia_runs(D.off);
     D.ldn = (n + DIA_ALIGN - 1) / DIA_ALIGN * DIA_ALIGN;
     // diagonal index by offset (reusing the count array): -1 for the others
     std::fill(cnt.begin(), cnt.end(), -1);

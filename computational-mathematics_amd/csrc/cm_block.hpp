@@ -201,9 +201,9 @@ __device__ __forceinline__ void block_colgroup_partials(double (&acc)[CPL][K > 0
         }
 }
 
-// Short rows, rows from the list rows[0..cnt): LPRW lanes per row, each
-// with CPL columns; the row's nonzeros in order, four index / value loads
-// issued ahead of their x loads.
+// Short rows, rows from the list rows[0..cnt) (rows null: rows 0..cnt-1, no
+// list to read): LPRW lanes per row, each with CPL columns; the row's
+// nonzeros in order, four index / value loads issued ahead of their x loads.
 template <class V, class T, class Acc, int KB, class Epi>
 __global__ __launch_bounds__(BS) void spmm_short(int cnt, const int *__restrict__ rows, const int *__restrict__ ptr,
                                                  const int *__restrict__ col, const V *__restrict__ val,
@@ -225,7 +225,7 @@ __global__ __launch_bounds__(BS) void spmm_short(int cnt, const int *__restrict_
     const int chunk = ((cnt + (int)gridDim.x - 1) / (int)gridDim.x + SR - 1) / SR * SR;
     const int q_begin = xcd_block(blockIdx.x, gridDim.x) * chunk, q_end = min(cnt, q_begin + chunk);
     for (int q = q_begin + threadIdx.x / LPRW; q < q_end; q += SR) {
-        const int r = rows[q];
+        const int r = rows ? rows[q] : q;
         Acc s[CPL];
 #pragma unroll
         for (int i = 0; i < CPL; i++) s[i] = 0;

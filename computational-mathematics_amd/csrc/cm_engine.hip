@@ -308,17 +308,16 @@ struct DiaIndex {
     // remainders are cut into ntile column tiles of tw columns and stored as a
     // CSR of segments, tile-major: segment t * nlong + q holds row q's entries
     // with columns in [t * tw, (t + 1) * tw) (lptr / lcol, values gathered
-    // through lsrc).  lrows = 0..nseg-1 (the segment list of the short-row
-    // kernel that sums them).  Walking the segments in order keeps the x rows
-    // that an XCD's waves in flight gather within about one tile, which its
-    // L2 holds.
+    // through lsrc); the short-row kernel sums them, segment q as its row q.
+    // Walking the segments in order keeps the x rows that an XCD's waves in
+    // flight gather within about one tile, which its L2 holds.
     int nlong = 0, ntile = 1, tw = 0;
     long long nlrem = 0, nseg = 0;
-    int *lpos = nullptr, *lptr = nullptr, *lcol = nullptr, *lsrc = nullptr, *lrows = nullptr;
+    int *lpos = nullptr, *lptr = nullptr, *lcol = nullptr, *lsrc = nullptr;
     size_t bytes() const
     {
         return ((size_t)nd + (size_t)nd * ldn + (rptr ? (size_t)n + 1 : 0) + 2 * (size_t)nrem +
-                (nlong ? (size_t)n + 2 * (size_t)nseg + 1 + 2 * (size_t)nlrem : 0)) *
+                (nlong ? (size_t)n + (size_t)nseg + 1 + 2 * (size_t)nlrem : 0)) *
                sizeof(int);
     }
     ~DiaIndex()
@@ -332,7 +331,6 @@ struct DiaIndex {
         (void)hipFree(lptr);
         (void)hipFree(lcol);
         (void)hipFree(lsrc);
-        (void)hipFree(lrows);
     }
 };
 
@@ -851,14 +849,10 @@ static int build_dia(DevCsr<double> &m, hipStream_t s)
     D->nseg = L.nseg;
     D->nlrem = (long long)L.lcol.size();
     if (D->nlong) {
-        std::vector<int> iota((size_t)L.nseg);
-        for (long long g = 0; g < L.nseg; g++) iota[g] = (int)g;
         CM_TRY(dev_upload(&D->lpos, L.lpos.data(), L.lpos.size(), s));
         CM_TRY(dev_upload(&D->lptr, L.lptr.data(), L.lptr.size(), s));
         CM_TRY(dev_upload(&D->lcol, L.lcol.data(), L.lcol.size(), s));
         CM_TRY(dev_upload(&D->lsrc, L.lsrc.data(), L.lsrc.size(), s));
-        CM_TRY(dev_upload(&D->lrows, iota.data(), iota.size(), s));
-        CM_HIP(hipStreamSynchronize(s));  // host temporaries
     }
     CM_TRY(dev_upload(&D->off, L.off.data(), L.off.size(), s));
     CM_TRY(dev_upload(&D->dsrc, L.dsrc.data(), L.dsrc.size(), s));
@@ -1813,7 +1807,8 @@ static int make_bins(RowBins &b, const DevCsr<V> &m, hipStream_t s, int short_ma
     b.short_max = short_max;
     b.ns = (int)sh.size();
     b.nl = (int)lo.size();
-    CM_TRY(dev_upload(&b.shortr, sh.data(), sh.size(), s));
+    // every row short: the list is 0..n-1 and the kernel takes q for rows[q]
+    if (b.nl > 0) CM_TRY(dev_upload(&b.shortr, sh.data(), sh.size(), s));
     CM_TRY(dev_upload(&b.longr, lo.data(), lo.size(), s));
     CM_HIP(hipStreamSynchronize(s));
     return CM_OK;
@@ -1987,7 +1982,7 @@ struct BlockKrylov {
                 const int gseg = (int)std::min<long long>((nseg + SROWS - 1) / SROWS, SPMM_CAP);
                 T *seg = D.ntile > 1 ? racc + (size_t)D.nlong * KB : racc;
                 spmm_short<V, T, Acc, KB, BEpiStore<T>><<<gseg, BS, 0, stream>>>(
-                    (int)nseg, D.lrows, D.lptr, D.lcol, m.lval, nullptr, xin, BEpiStore<T>{seg}, gf, partials, 0);
+                    (int)nseg, nullptr, D.lptr, D.lcol, m.lval, nullptr, xin, BEpiStore<T>{seg}, gf, partials, 0);
                 if (D.ntile > 1) {
                     const long long ne_l = (long long)D.nlong * KB;
                     const int gt = (int)std::min<long long>((ne_l + BS - 1) / BS, 8192);
@@ -2044,14 +2039,15 @@ struct BlockKrylov {
         *avg_us = 1e3 * ms / reps;
         if (m.block_dia && m.dval) {  // diagonals (zeros included) and remainder once, X and Y once
             const DiaIndex &D = *m.dia;
-            // (+ lpos, the segment pointers and list, and the segment sums
+            // (+ lpos, the segment pointers, and the segment sums
             // written and read back when the long remainders are tiled)
             *bytes = (double)D.nd * m.n * sizeof(V) + 4.0 * D.nd + (double)D.nrem * (sizeof(V) + sizeof(int)) +
-                     (D.rptr ? 4.0 * (m.n + 1) : 0.0) + (D.nlong ? 4.0 * m.n + 8.0 * D.nseg : 0.0) +
+                     (D.rptr ? 4.0 * (m.n + 1) : 0.0) + (D.nlong ? 4.0 * m.n + 4.0 * D.nseg : 0.0) +
                      (D.ntile > 1 ? 2.0 * D.nseg * KB * sizeof(T) : 0.0) +
                      2.0 * (double)ne() * sizeof(T) + (m.rowscale ? 4.0 * m.n : 0.0);
         } else {
-            *bytes = (double)m.nnz * (sizeof(V) + sizeof(int)) + 4.0 * (m.n + 1) + 4.0 * (bn.ns + bn.nl) +
+            *bytes = (double)m.nnz * (sizeof(V) + sizeof(int)) + 4.0 * (m.n + 1) +
+                     4.0 * ((bn.shortr ? bn.ns : 0) + bn.nl) +
                      2.0 * (double)ne() * sizeof(T) + (m.rowscale ? 4.0 * m.n : 0.0);
         }
         CM_HIP(hipGetLastError());

@@ -346,6 +346,11 @@ enum SpmvFormat { FMT_STAGED = 0, FMT_STAGED16 = 1, FMT_SLOT = 2, FMT_SLOT16 = 3
 // diagonals (the 17 of the 128^3 PDFIIM H factor: 3.7%), slower on few.
 template <class V>
 constexpr int dia_rpt() { return std::is_same<V, __half>::value ? 2 : 1; }
+// The second diagonal variant is offered (to the autotuner, and picked by a
+// one-shot call) from this many diagonals up: on fewer (C2's and C3's A, 7)
+// it timed within noise of the first in isolation and 1.5 µs slower in the
+// solve, so the autotuner's pick flipped from run to run.
+constexpr int DIA2_MIN_ND = 12;
 
 // A CSR matrix in HBM with its SpMV row blocks (and, for triangular factors
 // used by the level-scheduled solves, its level schedule).  A low-precision
@@ -1759,6 +1764,7 @@ static int tune_formats(KI &ki, int reps, hipEvent_t e0, hipEvent_t e1)
         float best = 1e30f;
         int pick = FMT_STAGED;
         for (int f = FMT_STAGED; f <= FMT_DIA2; f++) {
+            if (f == FMT_DIA2 && m.dia && m.dia->nd < DIA2_MIN_ND) continue;
             if (set_format(m, f, ki.stream) != CM_OK) continue;
             float ms = 0.f;
             CM_TRY(time_product(ki, m, reps, e0, e1, &ms));
@@ -3803,11 +3809,10 @@ int cm_system_kernel_format_bytes(cm_system h, int category, double *total_bytes
 // Formats of a one-shot system without timing anything: the diagonal format
 // (built on the device) for each SpMV matrix whose diagonals held by 60% of
 // the rows leave at most ONESHOT_DIA_REM of its nonzeros to the remainder --
-// the second variant from ONESHOT_DIA2_ND diagonals up, where it times
+// the second variant from DIA2_MIN_ND diagonals up, where it times
 // faster -- the staged row blocks otherwise, also for a matrix the device
 // build leaves to the host (whose index would take longer than the solve).
 constexpr double ONESHOT_DIA_REM = 0.3;
-constexpr int ONESHOT_DIA2_ND = 12;
 static int oneshot_formats(System &S)
 {
     K64 &k = *S.k64;
@@ -3816,7 +3821,7 @@ static int oneshot_formats(System &S)
         bool done = false;
         if (!m->dia) CM_TRY(build_dia_device(*m, k.stream, &done));
         if (!m->dia || (double)m->dia->nrem > ONESHOT_DIA_REM * (double)m->nnz) continue;
-        CM_TRY(set_format(*m, m->dia->nd >= ONESHOT_DIA2_ND ? FMT_DIA2 : FMT_DIA, k.stream));
+        CM_TRY(set_format(*m, m->dia->nd >= DIA2_MIN_ND ? FMT_DIA2 : FMT_DIA, k.stream));
         CM_TRY(k.ensure_partials((m->n + BS - 1) / BS));
     }
     return CM_OK;

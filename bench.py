@@ -881,8 +881,8 @@ def bench_single(args, w):
                      (", true>" if fmt == "slot16" else ", false>"))
         elif fmt == "dia":  # diagonals + CSR remainder: spmv_dia<V, X, Acc, Epi, CHUNK 8, 1 row per thread>
             kname = kname.replace("cm::spmv_kernel<2048, ", "cm::spmv_dia<")[:-2] + ", 8, 1>"
-        elif fmt == "dia2":  # second variant: 6 diagonals in flight, 2 rows per thread
-            kname = kname.replace("cm::spmv_kernel<2048, ", "cm::spmv_dia<")[:-2] + ", 6, 2>"
+        elif fmt == "dia2":  # second variant: 9 diagonals in flight, 2 rows per thread
+            kname = kname.replace("cm::spmv_kernel<2048, ", "cm::spmv_dia<")[:-2] + ", 9, 2>"
         else:
             tile = {"A": tuned["A"], "F1": tuned["GT"] or tuned["H"], "F2": tuned["G"]}.get(dom)
             if tile:

@@ -1235,7 +1235,7 @@ struct Krylov {
             const int nb = m.dia2 ? (m.n + BS * 2 * RPT - 1) / (BS * 2 * RPT) : (m.n + BS * RPT - 1) / (BS * RPT);
             timed(cat, alg_bytes<V, Epi>(m), [&] {
                 if (m.dia2)
-                    spmv_dia<V, T, Acc, Epi, 6, 2 * RPT><<<nb, BS, 0, stream>>>(
+                    spmv_dia<V, T, Acc, Epi, 9, 2 * RPT><<<nb, BS, 0, stream>>>(
                         m.n, D.ldn, D.nd, D.off, m.dval, D.rptr, D.rcol, m.rval, m.rowscale, xin, epi, gate, partials);
                 else
                     spmv_dia<V, T, Acc, Epi, 8, RPT><<<nb, BS, 0, stream>>>(

@@ -850,6 +850,14 @@ def bench_single(args, w):
             mixed[f"fp{bits}_factors"] = mixed_leg(sys_, bits, w, A_full, b, x, args)
             xs_mixed[f"fp{bits}_factors"] = sys_.get_x()
     formats = sys_.spmv_formats() if tuned else {}
+    # remainder entries per row of each matrix's diagonal index: the diagonal kernel stages the
+    # remainder through LDS from DIA_REM_LDS_MIN (0.1) per row up, a template argument of its name
+    rem_per_row = {}
+    for mat in ("A", "G", "H", "GT"):
+        try:
+            rem_per_row[mat] = sys_.dia_info(mat)[1] / A.n
+        except cmamd.EngineError:  # no such matrix in this system
+            pass
     sys_.close()
 
     value = args.steps / elapsed
@@ -879,10 +887,12 @@ def bench_single(args, w):
         if fmt.startswith("slot"):  # slot-major kernel: spmv_slot<V, X, Acc, Epi, 16-bit offsets>
             kname = (kname.replace("cm::spmv_kernel<2048, ", "cm::spmv_slot<")[:-2] +
                      (", true>" if fmt == "slot16" else ", false>"))
-        elif fmt == "dia":  # diagonals + CSR remainder: spmv_dia<V, X, Acc, Epi, CHUNK 8, 1 row per thread>
-            kname = kname.replace("cm::spmv_kernel<2048, ", "cm::spmv_dia<")[:-2] + ", 8, 1>"
-        elif fmt == "dia2":  # second variant: 9 diagonals in flight, 2 rows per thread
-            kname = kname.replace("cm::spmv_kernel<2048, ", "cm::spmv_dia<")[:-2] + ", 9, 2>"
+        elif fmt in ("dia", "dia2"):
+            # diagonals + CSR remainder: spmv_dia<V, X, Acc, Epi, CHUNK, rows per thread, remainder via LDS>;
+            # the second variant has 9 diagonals in flight and 2 rows per thread
+            lds = "true" if rem_per_row.get(mat, 0.0) >= 0.1 else "false"
+            shape = "8, 1" if fmt == "dia" else "9, 2"
+            kname = kname.replace("cm::spmv_kernel<2048, ", "cm::spmv_dia<")[:-2] + f", {shape}, {lds}>"
         else:
             tile = {"A": tuned["A"], "F1": tuned["GT"] or tuned["H"], "F2": tuned["G"]}.get(dom)
             if tile:

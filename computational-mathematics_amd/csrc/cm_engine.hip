@@ -1233,13 +1233,19 @@ struct Krylov {
             const DiaIndex &D = *m.dia;
             constexpr int RPT = dia_rpt<V>();
             const int nb = m.dia2 ? (m.n + BS * 2 * RPT - 1) / (BS * 2 * RPT) : (m.n + BS * RPT - 1) / (BS * RPT);
+            const bool rem_lds = (double)D.nrem >= DIA_REM_LDS_MIN * (double)m.n;
             timed(cat, alg_bytes<V, Epi>(m), [&] {
-                if (m.dia2)
-                    spmv_dia<V, T, Acc, Epi, 9, 2 * RPT><<<nb, BS, 0, stream>>>(
-                        m.n, D.ldn, D.nd, D.off, m.dval, D.rptr, D.rcol, m.rval, m.rowscale, xin, epi, gate, partials);
-                else
-                    spmv_dia<V, T, Acc, Epi, 8, RPT><<<nb, BS, 0, stream>>>(
-                        m.n, D.ldn, D.nd, D.off, m.dval, D.rptr, D.rcol, m.rval, m.rowscale, xin, epi, gate, partials);
+                auto go = [&](auto k) {
+                    k<<<nb, BS, 0, stream>>>(m.n, D.ldn, D.nd, D.off, m.dval, D.rptr, D.rcol, m.rval, m.rowscale, xin,
+                                             epi, gate, partials);
+                };
+                if (m.dia2) {
+                    if (rem_lds) go(spmv_dia<V, T, Acc, Epi, 9, 2 * RPT, true>);
+                    else go(spmv_dia<V, T, Acc, Epi, 9, 2 * RPT, false>);
+                } else {
+                    if (rem_lds) go(spmv_dia<V, T, Acc, Epi, 8, RPT, true>);
+                    else go(spmv_dia<V, T, Acc, Epi, 8, RPT, false>);
+                }
             }, spmv_bytes<V, Epi>(m));
             if constexpr (Epi::K > 0)
                 timed(KC_FIN, 8.0 * Epi::K * nb,

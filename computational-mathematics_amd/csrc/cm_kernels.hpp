@@ -723,24 +723,67 @@ template <>
 __device__ __forceinline__ float dia_val<__half, float>(__half v) { return __half2float(v); }
 
 // The remainder entries of a workgroup's rows (consecutive rows) are one
-// contiguous range [rs, re) of the remainder arrays.  When it holds at most
-// DIA_REM_LDS entries, every thread of the workgroup loads a share of it
-// (columns and values, coalesced) before the diagonals, gathers x for it after
-// them and writes the products to LDS; each row's thread then adds its own
-// products in column order.  A longer range falls back to each thread walking
-// its rows' entries.
+// contiguous range [rs, re) of the remainder arrays.  With REM_LDS and at
+// most DIA_REM_LDS entries in the range, every thread of the workgroup loads
+// a share of it (columns and values, coalesced) before the diagonals, gathers
+// x for it after them and writes the products to LDS; each row's thread then
+// adds its own products in column order.  Otherwise each thread walks its
+// rows' entries after the diagonals.  The host takes REM_LDS for a remainder
+// of at least DIA_REM_LDS_MIN entries per row: on a sparser one most waves
+// have no remainder row, and the workgroup-wide gather and barrier cost more
+// than the few rows' own walks.
 constexpr int DIA_REM_LDS = 1024;
+constexpr double DIA_REM_LDS_MIN = 0.1;
 
-template <class V, class X, class Acc, class Epi, int CHUNK, int RPT>
+// Adds rows i0 .. i0 + RPT - 1's diagonal products to s, diagonal by diagonal.
+template <class V, class X, class Acc, int CHUNK, int RPT>
+__device__ __forceinline__ void dia_rows(int n, int ldn, int nd, const int *__restrict__ off,
+                                         const V *__restrict__ dval, const X *__restrict__ x, int i0, Acc (&s)[RPT])
+{
+#pragma clang fp contract(off)
+    using VT = typename DiaVec<V, RPT>::type;
+    for (int d0 = 0; d0 < nd; d0 += CHUNK) {
+        VT v[CHUNK];
+        Acc xv[CHUNK][RPT];
+#pragma unroll
+        for (int q = 0; q < CHUNK; q++) {
+            const int d = d0 + q;
+#pragma unroll
+            for (int r = 0; r < RPT; r++) xv[q][r] = 0;
+            if (d < nd) {
+                const int c = i0 + off[d];
+                v[q] = __builtin_nontemporal_load(reinterpret_cast<const VT *>(dval + (size_t)d * ldn + i0));
+#pragma unroll
+                for (int r = 0; r < RPT; r++)
+                    if ((unsigned)(c + r) < (unsigned)n) xv[q][r] = (Acc)x[c + r];
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < CHUNK; q++)
+            if (d0 + q < nd) {
+#pragma unroll
+                for (int r = 0; r < RPT; r++) {
+                    V e;
+                    if constexpr (RPT == 1) {
+                        __builtin_memcpy(&e, &v[q], sizeof(V));
+                    } else {
+                        const auto lane = v[q][r];
+                        __builtin_memcpy(&e, &lane, sizeof(V));
+                    }
+                    s[r] += dia_val<V, Acc>(e) * xv[q][r];
+                }
+            }
+    }
+}
+
+template <class V, class X, class Acc, class Epi, int CHUNK, int RPT, bool REM_LDS>
 __global__ __launch_bounds__(BS) void spmv_dia(int n, int ldn, int nd, const int *__restrict__ off,
                                                const V *__restrict__ dval, const int *__restrict__ rptr,
                                                const int *__restrict__ rcol, const V *__restrict__ rval,
                                                const float *__restrict__ rowscale, const X *__restrict__ x, Epi epi,
                                                SolveState *st, double *partials)
 {
-#pragma clang fp contract(off)
     __shared__ double red[(Epi::K > 0 ? Epi::K : 1) * BS / WAVE];
-    __shared__ Acc prod[DIA_REM_LDS];
     double acc[Epi::K > 0 ? Epi::K : 1];
 #pragma unroll
     for (int k = 0; k < (Epi::K > 0 ? Epi::K : 1); k++) acc[k] = 0.0;
@@ -748,93 +791,84 @@ __global__ __launch_bounds__(BS) void spmv_dia(int n, int ldn, int nd, const int
     const int bi = xcd_block(blockIdx.x, gridDim.x);
     const int i0 = (bi * BS + (int)threadIdx.x) * RPT;
     if (st && st->done) return;
-    // the workgroup's remainder range, and this thread's share of it
-    const int r0 = bi * BS * RPT;
-    int rs = 0, re = 0;
-    if (rptr) {
-        rs = rptr[r0];
-        re = rptr[min(r0 + BS * RPT, n)];
-    }
-    const bool rem_lds = re - rs <= DIA_REM_LDS;
-    constexpr int PER = DIA_REM_LDS / BS;
-    int pc[PER];
-    Acc pv[PER];
+    if constexpr (!REM_LDS) {
+        if (i0 < n) {
+#pragma clang fp contract(off)
+            Acc s[RPT];
 #pragma unroll
-    for (int j = 0; j < PER; j++) {
-        pc[j] = -1;
-        pv[j] = 0;
-    }
-    if (rem_lds) {
+            for (int r = 0; r < RPT; r++) s[r] = 0;
+            // the rows' remainder bounds, loaded ahead of the diagonals so their
+            // latency hides behind the diagonal loads
+            int rb[RPT + 1];
+#pragma unroll
+            for (int r = 0; r <= RPT; r++) rb[r] = rptr ? rptr[min(i0 + r, n)] : 0;
+            dia_rows<V, X, Acc, CHUNK, RPT>(n, ldn, nd, off, dval, x, i0, s);
+#pragma unroll
+            for (int r = 0; r < RPT; r++) {
+                const int i = i0 + r;
+                if (i >= n) break;
+                Acc t = s[r];
+                for (int k = rb[r]; k < rb[r + 1]; k++) t += load_val<V, Acc>(rval, k) * (Acc)x[rcol[k]];
+                if (rowscale) t *= (Acc)rowscale[i];
+                epi.row(i, t, acc);
+            }
+        }
+    } else {
+#pragma clang fp contract(off)
+        __shared__ Acc prod[DIA_REM_LDS];
+        // the workgroup's remainder range, and this thread's share of it
+        const int r0 = bi * BS * RPT;
+        int rs = 0, re = 0;
+        if (rptr) {
+            rs = rptr[r0];
+            re = rptr[min(r0 + BS * RPT, n)];
+        }
+        const bool rem_lds = re - rs <= DIA_REM_LDS;
+        constexpr int PER = DIA_REM_LDS / BS;
+        int pc[PER];
+        Acc pv[PER];
 #pragma unroll
         for (int j = 0; j < PER; j++) {
-            const int k = rs + j * BS + (int)threadIdx.x;
-            if (k < re) {
-                pc[j] = rcol[k];
-                pv[j] = load_val<V, Acc>(rval, k);
-            }
+            pc[j] = -1;
+            pv[j] = 0;
         }
-    }
-    // the rows' remainder bounds
-    int rb[RPT + 1];
+        if (rem_lds) {
 #pragma unroll
-    for (int r = 0; r <= RPT; r++) rb[r] = rptr ? rptr[min(i0 + r, n)] : 0;
-    Acc s[RPT];
-#pragma unroll
-    for (int r = 0; r < RPT; r++) s[r] = 0;
-    if (i0 < n) {
-        using VT = typename DiaVec<V, RPT>::type;
-        for (int d0 = 0; d0 < nd; d0 += CHUNK) {
-            VT v[CHUNK];
-            Acc xv[CHUNK][RPT];
-#pragma unroll
-            for (int q = 0; q < CHUNK; q++) {
-                const int d = d0 + q;
-#pragma unroll
-                for (int r = 0; r < RPT; r++) xv[q][r] = 0;
-                if (d < nd) {
-                    const int c = i0 + off[d];
-                    v[q] = __builtin_nontemporal_load(reinterpret_cast<const VT *>(dval + (size_t)d * ldn + i0));
-#pragma unroll
-                    for (int r = 0; r < RPT; r++)
-                        if ((unsigned)(c + r) < (unsigned)n) xv[q][r] = (Acc)x[c + r];
+            for (int j = 0; j < PER; j++) {
+                const int k = rs + j * BS + (int)threadIdx.x;
+                if (k < re) {
+                    pc[j] = rcol[k];
+                    pv[j] = load_val<V, Acc>(rval, k);
                 }
             }
-#pragma unroll
-            for (int q = 0; q < CHUNK; q++)
-                if (d0 + q < nd) {
-#pragma unroll
-                    for (int r = 0; r < RPT; r++) {
-                        V e;
-                        if constexpr (RPT == 1) {
-                            __builtin_memcpy(&e, &v[q], sizeof(V));
-                        } else {
-                            const auto lane = v[q][r];
-                            __builtin_memcpy(&e, &lane, sizeof(V));
-                        }
-                        s[r] += dia_val<V, Acc>(e) * xv[q][r];
-                    }
-                }
         }
-    }
-    if (rem_lds && re > rs) {  // uniform over the workgroup
+        int rb[RPT + 1];
 #pragma unroll
-        for (int j = 0; j < PER; j++)
-            if (pc[j] >= 0) prod[j * BS + (int)threadIdx.x] = pv[j] * (Acc)x[pc[j]];
-        __syncthreads();
-    }
-    if (i0 < n) {
+        for (int r = 0; r <= RPT; r++) rb[r] = rptr ? rptr[min(i0 + r, n)] : 0;
+        Acc s[RPT];
 #pragma unroll
-        for (int r = 0; r < RPT; r++) {
-            const int i = i0 + r;
-            if (i >= n) break;
-            Acc t = s[r];
-            if (rem_lds) {
-                for (int k = rb[r]; k < rb[r + 1]; k++) t += prod[k - rs];
-            } else {
-                for (int k = rb[r]; k < rb[r + 1]; k++) t += load_val<V, Acc>(rval, k) * (Acc)x[rcol[k]];
+        for (int r = 0; r < RPT; r++) s[r] = 0;
+        if (i0 < n) dia_rows<V, X, Acc, CHUNK, RPT>(n, ldn, nd, off, dval, x, i0, s);
+        if (rem_lds && re > rs) {  // uniform over the workgroup
+#pragma unroll
+            for (int j = 0; j < PER; j++)
+                if (pc[j] >= 0) prod[j * BS + (int)threadIdx.x] = pv[j] * (Acc)x[pc[j]];
+            __syncthreads();
+        }
+        if (i0 < n) {
+#pragma unroll
+            for (int r = 0; r < RPT; r++) {
+                const int i = i0 + r;
+                if (i >= n) break;
+                Acc t = s[r];
+                if (rem_lds) {
+                    for (int k = rb[r]; k < rb[r + 1]; k++) t += prod[k - rs];
+                } else {
+                    for (int k = rb[r]; k < rb[r + 1]; k++) t += load_val<V, Acc>(rval, k) * (Acc)x[rcol[k]];
+                }
+                if (rowscale) t *= (Acc)rowscale[i];
+                epi.row(i, t, acc);
             }
-            if (rowscale) t *= (Acc)rowscale[i];
-            epi.row(i, t, acc);
         }
     }
     if constexpr (Epi::K > 0) block_partials<Epi::K>(acc, partials, red, bi);

@@ -3,8 +3,9 @@ against the selection rule restated here in numpy: the diagonals held by at
 least 60% of the rows, the 32 most populated (ties: the smaller offset), in
 offset order, every other entry in the CSR remainder.  The matrices cover
 stencils (all entries on diagonals), a factor with a remainder, a partly
-filled diagonal on each side of the 60% bar, more than 32 qualifying
-diagonals, nothing qualifying, and the two cases the device leaves to the
+filled diagonal on each side of the 60% bar, a remainder too dense to stage
+through LDS, more than 32 qualifying diagonals, nothing qualifying, and the
+two cases the device leaves to the
 host build (a sampled row longer than 64 entries, a remainder row longer
 than 64), whose index must follow the same rule.  Products through the
 index match the host CSR product, and a one-shot solve (which picks the
@@ -78,6 +79,9 @@ def _cases():
         "conv2d": cmamd.gen_convdiff(2, 50, 0.5, seed=3),
         "pdfiim_H": F3.H,
         "partial_fill": _with_diag(n, rng, {1: 1.0, -1: 1.0, 5: 0.7, -7: 0.55, 300: 0.62}, scatter=3000),
+        # ~8 remainder entries per row: a workgroup's remainder range exceeds what the product stages
+        # through LDS, so its rows walk their own entries
+        "dense_remainder": _with_diag(n, rng, {1: 1.0, -1: 1.0}, scatter=8 * n),
         "band40": band,
         "nothing": _csr(sp.random(n, n, 4e-3, random_state=5)),
         "long_sample_row": long_sample,

@@ -723,15 +723,17 @@ template <>
 __device__ __forceinline__ float dia_val<__half, float>(__half v) { return __half2float(v); }
 
 // The remainder entries of a workgroup's rows (consecutive rows) are one
-// contiguous range [rs, re) of the remainder arrays.  With REM_LDS and at
-// most DIA_REM_LDS entries in the range, every thread of the workgroup loads
-// a share of it (columns and values, coalesced) before the diagonals, gathers
-// x for it after them and writes the products to LDS; each row's thread then
-// adds its own products in column order.  Otherwise each thread walks its
-// rows' entries after the diagonals.  The host takes REM_LDS for a remainder
-// of at least DIA_REM_LDS_MIN entries per row: on a sparser one most waves
-// have no remainder row, and the workgroup-wide gather and barrier cost more
-// than the few rows' own walks.
+// contiguous range [rs, re) of the remainder arrays.  With REM_LDS the
+// workgroup walks it in chunks of DIA_REM_LDS entries: every thread loads a
+// share of a chunk (columns and values, coalesced; the first chunk before the
+// diagonals), gathers x for it and writes the products to LDS, and each row's
+// thread adds the products of its own entries in the chunk, in column order,
+// so a long row's entries are gathered by the whole workgroup instead of one
+// after another by its thread.  Without REM_LDS each thread walks its rows'
+// entries after the diagonals.  The host takes REM_LDS for a remainder of at
+// least DIA_REM_LDS_MIN entries per row: on a sparser one most waves have no
+// remainder row, and the workgroup-wide gather and barrier cost more than the
+// few rows' own walks.
 constexpr int DIA_REM_LDS = 1024;
 constexpr double DIA_REM_LDS_MIN = 0.1;
 
@@ -816,32 +818,31 @@ __global__ __launch_bounds__(BS) void spmv_dia(int n, int ldn, int nd, const int
     } else {
 #pragma clang fp contract(off)
         __shared__ Acc prod[DIA_REM_LDS];
-        // the workgroup's remainder range, and this thread's share of it
+        constexpr int PER = DIA_REM_LDS / BS;
+        // the workgroup's remainder range
         const int r0 = bi * BS * RPT;
         int rs = 0, re = 0;
         if (rptr) {
             rs = rptr[r0];
             re = rptr[min(r0 + BS * RPT, n)];
         }
-        const bool rem_lds = re - rs <= DIA_REM_LDS;
-        constexpr int PER = DIA_REM_LDS / BS;
+        // this thread's share of the chunk [c0, c0 + DIA_REM_LDS)
         int pc[PER];
         Acc pv[PER];
-#pragma unroll
-        for (int j = 0; j < PER; j++) {
-            pc[j] = -1;
-            pv[j] = 0;
-        }
-        if (rem_lds) {
+        auto load_share = [&](int c0) {
+            const int c1 = min(re, c0 + DIA_REM_LDS);
 #pragma unroll
             for (int j = 0; j < PER; j++) {
-                const int k = rs + j * BS + (int)threadIdx.x;
-                if (k < re) {
+                const int k = c0 + j * BS + (int)threadIdx.x;
+                pc[j] = -1;
+                pv[j] = 0;
+                if (k < c1) {
                     pc[j] = rcol[k];
                     pv[j] = load_val<V, Acc>(rval, k);
                 }
             }
-        }
+        };
+        load_share(rs);
         int rb[RPT + 1];
 #pragma unroll
         for (int r = 0; r <= RPT; r++) rb[r] = rptr ? rptr[min(i0 + r, n)] : 0;
@@ -849,11 +850,17 @@ __global__ __launch_bounds__(BS) void spmv_dia(int n, int ldn, int nd, const int
 #pragma unroll
         for (int r = 0; r < RPT; r++) s[r] = 0;
         if (i0 < n) dia_rows<V, X, Acc, CHUNK, RPT>(n, ldn, nd, off, dval, x, i0, s);
-        if (rem_lds && re > rs) {  // uniform over the workgroup
+        for (int c0 = rs; c0 < re; c0 += DIA_REM_LDS) {  // uniform over the workgroup
+            const int c1 = min(re, c0 + DIA_REM_LDS);
+            if (c0 != rs) load_share(c0);
 #pragma unroll
             for (int j = 0; j < PER; j++)
                 if (pc[j] >= 0) prod[j * BS + (int)threadIdx.x] = pv[j] * (Acc)x[pc[j]];
             __syncthreads();
+#pragma unroll
+            for (int r = 0; r < RPT; r++)
+                for (int k = max(rb[r], c0); k < min(rb[r + 1], c1); k++) s[r] += prod[k - c0];
+            if (c1 < re) __syncthreads();  // the next chunk overwrites prod
         }
         if (i0 < n) {
 #pragma unroll
@@ -861,11 +868,6 @@ __global__ __launch_bounds__(BS) void spmv_dia(int n, int ldn, int nd, const int
                 const int i = i0 + r;
                 if (i >= n) break;
                 Acc t = s[r];
-                if (rem_lds) {
-                    for (int k = rb[r]; k < rb[r + 1]; k++) t += prod[k - rs];
-                } else {
-                    for (int k = rb[r]; k < rb[r + 1]; k++) t += load_val<V, Acc>(rval, k) * (Acc)x[rcol[k]];
-                }
                 if (rowscale) t *= (Acc)rowscale[i];
                 epi.row(i, t, acc);
             }

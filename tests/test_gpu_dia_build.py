@@ -8,7 +8,8 @@ through LDS, more than 32 qualifying diagonals, nothing qualifying, and the
 two cases the device leaves to the
 host build (a sampled row longer than 64 entries, a remainder row longer
 than 64), whose index must follow the same rule.  Products through the
-index match the host CSR product, and a one-shot solve (which picks the
+index, in both diagonal variants, match the host CSR product and each other
+bit for bit, and a one-shot solve (which picks the
 diagonal format from this index without timing) matches the oracle."""
 import numpy as np
 import pytest
@@ -101,11 +102,15 @@ def test_device_dia_index_follows_rule(name):
     assert np.array_equal(off, want_off), (off, want_off)
     if len(off):
         assert nrem == want_rem
-        s.spmv_format(64, "A", "dia")
         x = cmamd.gen_uniform(4, M.n)
-        y = s.spmv("A", x)
         ref = cmamd.csr_mv_host(M, x)
-        assert np.linalg.norm(y - ref) <= 1e-13 * np.linalg.norm(ref)
+        ys = []
+        for fmt in ("dia", "dia2"):  # one row per thread; two rows per thread, 9 diagonals in flight
+            s.spmv_format(64, "A", fmt)
+            ys.append(s.spmv("A", x))
+            assert np.linalg.norm(ys[-1] - ref) <= 1e-13 * np.linalg.norm(ref), fmt
+        # each row's sum runs in the same order in both variants
+        assert np.array_equal(ys[0], ys[1])
     s.close()
 
 

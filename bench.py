@@ -144,7 +144,7 @@ def pmc_block_summary(fetch_db, write_db, copy_fetch_db, copy_write_db, workload
         if not v:
             raise SystemExit(f"no 1 GiB copy dispatch in {db}")
         scale[name] = gib / (1024.0 * float(np.median(v)))
-    dia_k = "spmm_dia<float, float, float, 16, cm::BEpiStore<float> >"
+    dia_k = "spmm_dia<float, float, float, 16, cm::BEpiStore<float>"  # either remainder form
     long_k = "spmm_long<float, float, float, 16, cm::BEpiStore<float> >"
     seg_k, tiles_k = "spmm_short<float, float, float, 16, cm::BEpiStore<float> >", "k_long_tiles<float, float>"
     per = {}

@@ -261,6 +261,38 @@ __global__ __launch_bounds__(BS) void spmm_short(int cnt, const int *__restrict_
     block_colgroup_partials<Epi::K, KB, CPL>(acc, red, partials, slot0 + blockIdx.x);
 }
 
+// Adds row r's diagonal products over this lane's CPL columns to s,
+// CH diagonals in flight per step.
+template <class V, class T, class Acc, int KB, int CPL, int CH>
+__device__ __forceinline__ void spmm_dia_row(int n, int ldn, int nd, const int *__restrict__ off,
+                                             const V *__restrict__ dval, const T *__restrict__ x, int r, int c0,
+                                             Acc (&s)[CPL])
+{
+#pragma clang fp contract(off)
+    for (int d0 = 0; d0 < nd; d0 += CH) {
+        Acc v[CH];
+        T xv[CH][CPL];
+#pragma unroll
+        for (int q = 0; q < CH; q++) {
+            const int d = d0 + q;
+            v[q] = 0;
+#pragma unroll
+            for (int i = 0; i < CPL; i++) xv[q][i] = 0;
+            if (d < nd) {
+                const int c = r + off[d];
+                v[q] = load_val<V, Acc>(dval + (size_t)d * ldn, r);
+                if ((unsigned)c < (unsigned)n) ld_cols<T, CPL>(x + (size_t)c * KB + c0, xv[q]);
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < CH; q++)
+            if (d0 + q < nd) {
+#pragma unroll
+                for (int i = 0; i < CPL; i++) s[i] += v[q] * (Acc)xv[q][i];
+            }
+    }
+}
+
 // Diagonals plus a CSR remainder (the single-RHS spmv_dia layout: dval[d *
 // ldn + i] = A(i, i + off[d]), remainder rptr / rcol / rval, rptr null when
 // empty) over KB columns: LPRW lanes per row, each with CPL columns, rows
@@ -272,7 +304,13 @@ __global__ __launch_bounds__(BS) void spmm_short(int cnt, const int *__restrict_
 // >= 0; lpos null when there are none) take their remainder sums from racc
 // (KB values per long row, written before this launch from the remainders'
 // column-tiled segments) instead of walking the remainder themselves.
-template <class V, class T, class Acc, int KB, class Epi>
+// With REM_LDS the workgroup takes its SR rows at a time, and when their
+// remainder entries number at most 2 SR, each lane group loads one or two of
+// them (column and value) before the diagonals, gathers their x rows after
+// them and writes the KB-wide products to LDS, from which each row adds its
+// own in column order; otherwise each row walks its own entries.  The host
+// takes REM_LDS for a remainder of at least DIA_REM_LDS_MIN entries per row.
+template <class V, class T, class Acc, int KB, class Epi, bool REM_LDS>
 __global__ __launch_bounds__(BS) void spmm_dia(int n, int ldn, int nd, const int *__restrict__ off,
                                                const V *__restrict__ dval, const int *__restrict__ rptr,
                                                const int *__restrict__ rcol, const V *__restrict__ rval,
@@ -293,51 +331,103 @@ __global__ __launch_bounds__(BS) void spmm_dia(int n, int ldn, int nd, const int
     constexpr int SR = BS / LPRW;
     const int chunk = ((n + (int)gridDim.x - 1) / (int)gridDim.x + SR - 1) / SR * SR;
     const int r_begin = xcd_block(blockIdx.x, gridDim.x) * chunk, r_end = min(n, r_begin + chunk);
-    for (int r = r_begin + (int)threadIdx.x / LPRW; r < r_end; r += SR) {
+    if constexpr (!REM_LDS) {
+        for (int r = r_begin + (int)threadIdx.x / LPRW; r < r_end; r += SR) {
 #pragma clang fp contract(off)
-        Acc s[CPL];
+            Acc s[CPL];
 #pragma unroll
-        for (int i = 0; i < CPL; i++) s[i] = 0;
-        for (int d0 = 0; d0 < nd; d0 += CH) {
-            Acc v[CH];
-            T xv[CH][CPL];
-#pragma unroll
-            for (int q = 0; q < CH; q++) {
-                const int d = d0 + q;
-                v[q] = 0;
-#pragma unroll
-                for (int i = 0; i < CPL; i++) xv[q][i] = 0;
-                if (d < nd) {
-                    const int c = r + off[d];
-                    v[q] = load_val<V, Acc>(dval + (size_t)d * ldn, r);
-                    if ((unsigned)c < (unsigned)n) ld_cols<T, CPL>(x + (size_t)c * KB + c0, xv[q]);
-                }
-            }
-#pragma unroll
-            for (int q = 0; q < CH; q++)
-                if (d0 + q < nd) {
-#pragma unroll
-                    for (int i = 0; i < CPL; i++) s[i] += v[q] * (Acc)xv[q][i];
-                }
-        }
-        const int lq = lpos ? lpos[r] : -1;
-        if (lq >= 0) {
-            T xr[CPL];
-            ld_cols<T, CPL>(racc + (size_t)lq * KB + c0, xr);
-#pragma unroll
-            for (int i = 0; i < CPL; i++) s[i] += (Acc)xr[i];
-        } else if (rptr) {
-            for (int k = rptr[r]; k < rptr[r + 1]; k++) {
+            for (int i = 0; i < CPL; i++) s[i] = 0;
+            spmm_dia_row<V, T, Acc, KB, CPL, CH>(n, ldn, nd, off, dval, x, r, c0, s);
+            const int lq = lpos ? lpos[r] : -1;
+            if (lq >= 0) {
                 T xr[CPL];
-                ld_cols<T, CPL>(x + (size_t)rcol[k] * KB + c0, xr);
-                const Acc vk = load_val<V, Acc>(rval, k);
+                ld_cols<T, CPL>(racc + (size_t)lq * KB + c0, xr);
 #pragma unroll
-                for (int i = 0; i < CPL; i++) s[i] += vk * (Acc)xr[i];
+                for (int i = 0; i < CPL; i++) s[i] += (Acc)xr[i];
+            } else if (rptr) {
+                for (int k = rptr[r]; k < rptr[r + 1]; k++) {
+                    T xr[CPL];
+                    ld_cols<T, CPL>(x + (size_t)rcol[k] * KB + c0, xr);
+                    const Acc vk = load_val<V, Acc>(rval, k);
+#pragma unroll
+                    for (int i = 0; i < CPL; i++) s[i] += vk * (Acc)xr[i];
+                }
+            }
+            const Acc sc = rowscale ? (Acc)rowscale[r] : (Acc)1;
+#pragma unroll
+            for (int i = 0; i < CPL; i++)
+                epi.row((size_t)r * KB + c0 + i, c0 + i, rowscale ? s[i] * sc : s[i], acc[i]);
+        }
+    } else {
+#pragma clang fp contract(off)
+        constexpr int PER = 2;  // staged entries per lane group
+        __shared__ Acc prod[PER * SR * KB];
+        const int g = (int)threadIdx.x / LPRW;
+        for (int p0 = r_begin; p0 < r_end; p0 += SR) {  // uniform over the workgroup
+            const int r = p0 + g;
+            const bool live = r < r_end;
+            int q0 = 0, q1 = 0;
+            if (rptr) {
+                q0 = rptr[p0];
+                q1 = rptr[min(p0 + SR, r_end)];
+            }
+            const bool stage = q1 - q0 <= PER * SR;
+            int ec[PER];
+            Acc ev[PER];
+#pragma unroll
+            for (int j = 0; j < PER; j++) {
+                const int e = q0 + j * SR + g;
+                ec[j] = -1;
+                ev[j] = 0;
+                if (stage && e < q1) {
+                    ec[j] = rcol[e];
+                    ev[j] = load_val<V, Acc>(rval, e);
+                }
+            }
+            Acc s[CPL];
+#pragma unroll
+            for (int i = 0; i < CPL; i++) s[i] = 0;
+            if (live) spmm_dia_row<V, T, Acc, KB, CPL, CH>(n, ldn, nd, off, dval, x, r, c0, s);
+            const int lq = live && lpos ? lpos[r] : -1;
+            if (live && lq >= 0) {
+                T xr[CPL];
+                ld_cols<T, CPL>(racc + (size_t)lq * KB + c0, xr);
+#pragma unroll
+                for (int i = 0; i < CPL; i++) s[i] += (Acc)xr[i];
+            }
+            if (stage) {
+                if (q1 > q0) {
+#pragma unroll
+                    for (int j = 0; j < PER; j++)
+                        if (ec[j] >= 0) {
+                            T xr[CPL];
+                            ld_cols<T, CPL>(x + (size_t)ec[j] * KB + c0, xr);
+#pragma unroll
+                            for (int i = 0; i < CPL; i++) prod[(j * SR + g) * KB + c0 + i] = ev[j] * (Acc)xr[i];
+                        }
+                    __syncthreads();
+                    if (live && lq < 0)
+                        for (int k = rptr[r]; k < rptr[r + 1]; k++)
+#pragma unroll
+                            for (int i = 0; i < CPL; i++) s[i] += prod[(k - q0) * KB + c0 + i];
+                    __syncthreads();  // the next rows' products overwrite prod
+                }
+            } else if (live && lq < 0 && rptr) {
+                for (int k = rptr[r]; k < rptr[r + 1]; k++) {
+                    T xr[CPL];
+                    ld_cols<T, CPL>(x + (size_t)rcol[k] * KB + c0, xr);
+                    const Acc vk = load_val<V, Acc>(rval, k);
+#pragma unroll
+                    for (int i = 0; i < CPL; i++) s[i] += vk * (Acc)xr[i];
+                }
+            }
+            if (live) {
+                const Acc sc = rowscale ? (Acc)rowscale[r] : (Acc)1;
+#pragma unroll
+                for (int i = 0; i < CPL; i++)
+                    epi.row((size_t)r * KB + c0 + i, c0 + i, rowscale ? s[i] * sc : s[i], acc[i]);
             }
         }
-        const Acc sc = rowscale ? (Acc)rowscale[r] : (Acc)1;
-#pragma unroll
-        for (int i = 0; i < CPL; i++) epi.row((size_t)r * KB + c0 + i, c0 + i, rowscale ? s[i] * sc : s[i], acc[i]);
     }
     block_colgroup_partials<Epi::K, KB, CPL>(acc, red, partials, slot0 + blockIdx.x);
 }

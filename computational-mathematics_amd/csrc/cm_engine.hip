@@ -2001,9 +2001,12 @@ struct BlockKrylov {
                     k_long_tiles<T, Acc><<<gt, BS, 0, stream>>>(ne_l, D.ntile, seg, racc, gf);
                 }
             }
-            spmm_dia<V, T, Acc, KB, Epi><<<gd, BS, 0, stream>>>(n, D.ldn, D.nd, D.off, m.dval, D.rptr, D.rcol, m.rval,
-                                                               D.nlong ? D.lpos : nullptr, racc, m.rowscale, xin,
-                                                               epi, gf, partials, 0);
+            auto go = [&](auto k) {
+                k<<<gd, BS, 0, stream>>>(n, D.ldn, D.nd, D.off, m.dval, D.rptr, D.rcol, m.rval,
+                                         D.nlong ? D.lpos : nullptr, racc, m.rowscale, xin, epi, gf, partials, 0);
+            };
+            if ((double)D.nrem >= DIA_REM_LDS_MIN * (double)n) go(spmm_dia<V, T, Acc, KB, Epi, true>);
+            else go(spmm_dia<V, T, Acc, KB, Epi, false>);
             if constexpr (Epi::K > 0) finalize<Epi::K>(gd, epi, fst, gate, gate ? 1 : 0);
             return;
         }

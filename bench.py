@@ -275,15 +275,16 @@ def mixed_leg(sys_, bits, w, A, b, x64, args):
     vectors, to the same true relative residual as the FP64 path."""
     import cmamd
     sys_.enable_mixed(bits)
+    itol = args.inner_tol
     for _ in range(max(1, args.warmup)):
         sys_.set_x(None)
-        sys_.solve_mixed(bits, tol=w["tol"], inner_tol=1e-4, nmax=w["nmax"])
+        sys_.solve_mixed(bits, tol=w["tol"], inner_tol=itol, nmax=w["nmax"])
     lat = []
     t0 = time.perf_counter()
     for _ in range(args.steps):
         ts = time.perf_counter()
         sys_.set_x(None)
-        rv, outer, inner, info = sys_.solve_mixed(bits, tol=w["tol"], inner_tol=1e-4, nmax=w["nmax"])
+        rv, outer, inner, info = sys_.solve_mixed(bits, tol=w["tol"], inner_tol=itol, nmax=w["nmax"])
         lat.append(time.perf_counter() - ts)
     elapsed = time.perf_counter() - t0
     x = sys_.get_x()
@@ -295,7 +296,7 @@ def mixed_leg(sys_, bits, w, A, b, x64, args):
             "outer_steps": outer, "inner_passes": inner, "info": info, "true_rel_residual": true_res,
             "rel_error_vs_fp64_solution": float(np.linalg.norm(x - x64) / np.linalg.norm(x64)),
             "inner": f"BiCGSTAB on FP32 A, FP{bits} factors{' (per-row scaled)' if bits == 16 else ''}, "
-                     f"FP32 vectors, to 1e-4; FP64 refinement of b - A x"}
+                     f"FP32 vectors, to {itol:g}; FP64 refinement of b - A x"}
 
 
 def _pct(v, q):
@@ -645,6 +646,8 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision legs")
+    ap.add_argument("--inner-tol", type=float, default=1e-4,
+                    help="relative residual the mixed legs' low-precision inner solves stop at")
     ap.add_argument("--kernel-timing", choices=("timed-region", "separate"), default="timed-region",
                     help="bracket the dominant kernel's launches with HIP events inside the timed region "
                          "(default), or time the region without events and time that kernel in extra solves")

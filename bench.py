@@ -765,7 +765,8 @@ def _leg_summary(out, wall_s):
                                               "avg_us", "format_frac") if k in r}
     if out.get("cpu_baseline"):
         c = out["cpu_baseline"]
-        leg["cpu_baseline"] = {k: c[k] for k in ("value", "unit", "cores", "kind", "sample") if k in c}
+        leg["cpu_baseline"] = {k: c[k] for k in ("value", "unit", "cores", "kind", "sample", "single_thread")
+                               if k in c}
     leg["wall_s"] = round(wall_s, 1)
     return leg
 

@@ -341,15 +341,15 @@ enum SpmvFormat { FMT_STAGED = 0, FMT_STAGED16 = 1, FMT_SLOT = 2, FMT_SLOT16 = 3
 
 // Rows per thread of spmv_dia: FP16 values are read two rows at a time (one
 // 4-byte load per diagonal); FP32 and FP64 one row per thread.  The second
-// diagonal variant (FMT_DIA2) doubles the rows per thread and keeps fewer
-// diagonals in flight (6 instead of 8): faster on matrices with many
-// diagonals (the 17 of the 128^3 PDFIIM H factor: 3.7%), slower on few.
+// diagonal variant (FMT_DIA2) doubles the rows per thread and takes 9
+// diagonals per chunk instead of 8: faster on matrices with many diagonals
+// (the 17 of the 128^3 PDFIIM H factor, in two chunks), slower on few.
 template <class V>
 constexpr int dia_rpt() { return std::is_same<V, __half>::value ? 2 : 1; }
 // The second diagonal variant is offered (to the autotuner, and picked by a
-// one-shot call) from this many diagonals up: on fewer (C2's and C3's A, 7)
-// it timed within noise of the first in isolation and 1.5 µs slower in the
-// solve, so the autotuner's pick flipped from run to run.
+// one-shot call) from this many diagonals up: on fewer (a 7-point stencil's
+// 7) it timed within noise of the first in isolation and 1.5 µs slower in
+// the solve, so the autotuner's pick flipped from run to run.
 constexpr int DIA2_MIN_ND = 12;
 
 // A CSR matrix in HBM with its SpMV row blocks (and, for triangular factors

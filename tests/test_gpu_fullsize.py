@@ -1,5 +1,5 @@
-"""Parity at the full sizes of the benchmark configurations (the C1 case, the
-2D 100^2 system, runs at its size in test_gpu_parity.py):
+"""Parity at the full sizes of the benchmark configurations (the 2D 100^2
+AFIIM system runs at its size in test_gpu_parity.py):
 
 * cd3d-128 -- 3D 7-point 128^3 (n = 2,097,152), Peclet 1, PDFIIM(2,2,1,0.05)
   (built on the GPU too, equal to the host producer's factors bit for bit),

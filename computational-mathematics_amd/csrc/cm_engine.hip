@@ -385,6 +385,7 @@ struct DevCsr {
     bool lower = false;  // the level schedule's triangle (lower: rows read smaller columns)
     int *lrows = nullptr;
     std::vector<int> loff;
+    int *dloff = nullptr;         // loff on the device (read by the run-of-levels kernel)
     std::vector<long long> lnnz;  // per level: the triangle's entries of its rows
     int lpr = 1;                  // lanes per row of the level kernel
     // the triangle's entries in level order (tptr indexed by level position;
@@ -406,6 +407,7 @@ struct DevCsr {
             if (slots) b += slots->bytes();
             if (dia) b += dia->bytes();
             b += (size_t)(n + 1 + 2 * (nblk + 1)) * sizeof(int) + (size_t)nnz * sizeof(int) + (lrows ? (size_t)n * sizeof(int) : 0);
+            if (dloff) b += (size_t)(nlev + 1) * sizeof(int);
             if (d16) b += (size_t)(nnz + 2 * CSR_PAD) * sizeof(uint16_t) + (size_t)nblk * sizeof(int);
         }
         return b;
@@ -417,6 +419,7 @@ struct DevCsr {
             (void)hipFree(col);
             (void)hipFree(blk);
             (void)hipFree(lrows);
+            (void)hipFree(dloff);
             (void)hipFree(d16);
             (void)hipFree(cbase);
             (void)hipFree(tptr);
@@ -430,7 +433,7 @@ struct DevCsr {
         (void)hipFree(dval);
         (void)hipFree(rval);
         (void)hipFree(lval);
-        ptr = col = blk = lrows = cbase = nullptr;
+        ptr = col = blk = lrows = cbase = dloff = nullptr;
         tptr = tcol = tsrc = nullptr;
         ntri = 0;
         d16 = nullptr;
@@ -513,6 +516,7 @@ static int cast_csr(DevCsr<V> &dst, const DevCsr<double> &src, hipStream_t s)
     dst.lower = src.lower;
     dst.lrows = src.lrows;
     dst.loff = src.loff;
+    dst.dloff = src.dloff;
     dst.lnnz = src.lnnz;
     dst.lpr = src.lpr;
     dst.tptr = src.tptr;
@@ -932,6 +936,7 @@ static int build_levels(DevCsr<double> &m, const int *ptr, const int *col, bool 
     m.lower = lower;
     m.loff = L.loff;
     CM_TRY(dev_upload(&m.lrows, L.rows.data(), (size_t)m.n, s));
+    CM_TRY(dev_upload(&m.dloff, L.loff.data(), L.loff.size(), s));
     // the triangle's entries in level order
     const int n = m.n;
     DevTemps tmp;
@@ -1310,31 +1315,56 @@ struct Krylov {
                   [&] { k_finalize<Epi::K, Epi><<<1, FBS, 0, stream>>>(m.nblk, partials, kst, epi); });
     }
 
-    // x = T^{-1} b for a unit triangular factor with its level schedule
+    // x = T^{-1} b for a unit triangular factor with its level schedule: one
+    // launch per level, or one workgroup for a run of at least two consecutive
+    // levels that each take at most TRSV_RUN_STEPS steps of its lane groups
     void trsv(const DevCsr<VF> &m, const T *bin, T *xout, bool use_state)
     {
-        for (int l = 0; l < m.nlev; l++) {
+        const SolveState *gate = use_state ? st : nullptr;
+        const int run_rows = TRSV_RUN_STEPS * (TRSV_RUN_THREADS / m.lpr);
+        auto small = [&](int l) { return m.loff[l + 1] - m.loff[l] <= run_rows; };
+        auto lanes = [&](auto go) {
+            switch (m.lpr) {
+            case 1: go(std::integral_constant<int, 1>{}); break;
+            case 2: go(std::integral_constant<int, 2>{}); break;
+            case 4: go(std::integral_constant<int, 4>{}); break;
+            case 8: go(std::integral_constant<int, 8>{}); break;
+            default: go(std::integral_constant<int, 16>{}); break;
+            }
+        };
+        // the levels' entries (value + column), row list and row pointers, and per row the
+        // right-hand side read, the solution written and (once over all levels) read back
+        auto level_bytes = [&](int l) {
             const int cnt = m.loff[l + 1] - m.loff[l];
-            // the level's entries (value + column), its row list and row pointers, and per row the
-            // right-hand side read, the solution written and (once over all levels) read back
-            const double by = (double)m.lnnz[l] * (sizeof(VF) + 4.0) + (double)cnt * (4.0 + 8.0 + 3.0 * sizeof(T));
-            timed(KC_LU, by, [&] {
-                const SolveState *gate = use_state ? st : nullptr;
-                const int q0 = m.loff[l];
-                auto go = [&](auto lanes) {
-                    constexpr int LPR = decltype(lanes)::value;
+            return (double)m.lnnz[l] * (sizeof(VF) + 4.0) + (double)cnt * (4.0 + 8.0 + 3.0 * sizeof(T));
+        };
+        for (int l = 0; l < m.nlev;) {
+            int l2 = l;
+            while (l2 < m.nlev && small(l2)) l2++;
+            if (l2 - l >= 2 && m.dloff) {
+                double by = 0.0;
+                for (int q = l; q < l2; q++) by += level_bytes(q);
+                timed(KC_LU, by, [&] {
+                    lanes([&](auto lz) {
+                        constexpr int LPR = decltype(lz)::value;
+                        k_trsv_run<T, VF, LPR><<<1, TRSV_RUN_THREADS, 0, stream>>>(
+                            l, l2, m.dloff, m.lrows, m.tptr, m.tcol, m.tval, m.rowscale, bin, xout, gate);
+                    });
+                });
+                l = l2;
+                continue;
+            }
+            const int cnt = m.loff[l + 1] - m.loff[l];
+            const int q0 = m.loff[l];
+            timed(KC_LU, level_bytes(l), [&] {
+                lanes([&](auto lz) {
+                    constexpr int LPR = decltype(lz)::value;
                     const int g = (int)(((long long)cnt * LPR + BS - 1) / BS);
                     k_trsv_level<T, VF, LPR><<<g, BS, 0, stream>>>(cnt, m.lrows + q0, m.tptr + q0, m.tcol, m.tval,
                                                                    m.rowscale, bin, xout, gate);
-                };
-                switch (m.lpr) {
-                case 1: go(std::integral_constant<int, 1>{}); break;
-                case 2: go(std::integral_constant<int, 2>{}); break;
-                case 4: go(std::integral_constant<int, 4>{}); break;
-                case 8: go(std::integral_constant<int, 8>{}); break;
-                default: go(std::integral_constant<int, 16>{}); break;
-                }
+                });
             });
+            l++;
         }
     }
 

@@ -2035,7 +2035,8 @@ struct BlockKrylov {
                 k<<<gd, BS, 0, stream>>>(n, D.ldn, D.nd, D.off, m.dval, D.rptr, D.rcol, m.rval,
                                          D.nlong ? D.lpos : nullptr, racc, m.rowscale, xin, epi, gf, partials, 0);
             };
-            if ((double)D.nrem >= DIA_REM_LDS_MIN * (double)n) go(spmm_dia<V, T, Acc, KB, Epi, true>);
+            // the remainder the kernel itself walks: long rows' entries arrive summed in racc
+            if ((double)(D.nrem - D.nlrem) >= DIA_REM_LDS_MIN * (double)n) go(spmm_dia<V, T, Acc, KB, Epi, true>);
             else go(spmm_dia<V, T, Acc, KB, Epi, false>);
             if constexpr (Epi::K > 0) finalize<Epi::K>(gd, epi, fst, gate, gate ? 1 : 0);
             return;

@@ -874,7 +874,7 @@ def bench_single(args, w):
     # remainder entries per row of each matrix's diagonal index: the diagonal kernel stages the
     # remainder through LDS from DIA_REM_LDS_MIN (0.1) per row up, a template argument of its name
     rem_per_row = {}
-    for mat in ("A", "G", "H", "GT"):
+    for mat in ("A", "G", "H", "GT", "L", "U"):
         try:
             rem_per_row[mat] = sys_.dia_info(mat)[1] / A.n
         except cmamd.EngineError:  # no such matrix in this system
@@ -898,8 +898,17 @@ def bench_single(args, w):
     kname = CAT_KERNELS.get(dom, dom)
     mode = w.get("mode", "E")
     if dom == "LU":  # mode C's L / U products, or modes I / J's level-scheduled triangular solves
-        kname = ("cm::k_trsv_level<double, double> (one launch per dependency level)" if mode in ("I", "J")
-                 else "cm::spmv_kernel<2048, double, double, double, cm::EpiSubFrom<double> >")
+        if mode in ("I", "J"):
+            kname = "cm::k_trsv_level<double, double> (one launch per dependency level)"
+        else:
+            fmt_l = formats.get(64, {}).get("L", "staged") if tuned else "staged"
+            if fmt_l in ("dia", "dia2"):
+                lds = "true" if rem_per_row.get("L", 0.0) >= 0.1 else "false"
+                shape = "8, 1" if fmt_l == "dia" else "9, 2"
+                kname = f"cm::spmv_dia<double, double, double, cm::EpiSubFrom<double>, {shape}, {lds}>"
+            else:
+                tile = (tuned or {}).get("L") or 2048
+                kname = f"cm::spmv_kernel<{tile}, double, double, double, cm::EpiSubFrom<double> >"
     elif mode == "C" and w.get("nr", 1) > 1 and dom in ("F1", "F2"):
         kname = kname.split(", cm::Epi")[0] + ", cm::EpiStore|EpiAccum|EpiAccumScale<double> >"
     if tuned and dom != "LU":

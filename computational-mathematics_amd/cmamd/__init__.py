@@ -543,9 +543,9 @@ class System:
 
     def dia_info(self, which="A"):
         """(offsets of the stored diagonals, remainder entries) of matrix
-        'A', 'G', 'H' or 'GT' (the diagonal index is built if absent; no
-        offsets when no diagonal is held by 60% of the rows)."""
-        q = ("A", "G", "H", "GT").index(which)
+        'A', 'G', 'H', 'GT', 'L' or 'U' (the diagonal index is built if
+        absent; no offsets when no diagonal is held by 60% of the rows)."""
+        q = ("A", "G", "H", "GT", "L", "U").index(which)
         nd, nrem = C.c_int(), C.c_longlong()
         off = np.zeros(32, np.int32)
         _check(self._lib.cm_system_dia_info(self._h, q, C.byref(nd), _ip(off), C.byref(nrem)))

@@ -1133,6 +1133,8 @@ struct Krylov {
 
     bool uses_gh() const { return variant != 2 || mode == 'E' || mode == 'C'; }
     bool uses_lu() const { return variant == 2 && ((mode == 'C' && nr > 1) || mode == 'I' || mode == 'J'); }
+    // L and U in sparse products (mode C's sweeps), not only in triangular solves
+    bool lu_spmv() const { return variant == 2 && mode == 'C' && nr > 1; }
 
     // vectors, state, partials, events
     int alloc_work(int nblk_a)
@@ -1787,8 +1789,9 @@ static int time_product(KI &ki, const DevCsr<V> &m, int reps, hipEvent_t e0, hip
     return CM_OK;
 }
 
-// Autotuner, per SpMV matrix of one Krylov (A, G, H, GT): every available
-// format timed (reps products each), the fastest kept.  The staged row
+// Autotuner, per SpMV matrix of one Krylov (A, G, H, GT; L and U in mode
+// C's sweeps): every available format timed (reps products each), the
+// fastest kept.  The staged row
 // blocks are the FP64 matrix's current ones.
 template <class KI>
 static int tune_formats(KI &ki, int reps, hipEvent_t e0, hipEvent_t e1)
@@ -1815,6 +1818,10 @@ static int tune_formats(KI &ki, int reps, hipEvent_t e0, hipEvent_t e1)
     CM_TRY(one(ki.G));
     CM_TRY(one(ki.H));
     CM_TRY(one(ki.GT));
+    if (ki.lu_spmv()) {
+        CM_TRY(one(ki.L));
+        CM_TRY(one(ki.U));
+    }
     ki.drop_graphs();
     return CM_OK;
 }
@@ -3248,9 +3255,10 @@ int cm_system_autotune(cm_system h, int reps, int *tiles)
         maxblk = std::max(maxblk, m.nblk);
         if (tiles) tiles[q] = best_t;
     }
-    // slot-major index of the SpMV matrices (A, G, H, GT), then the format
-    // per matrix and precision
-    for (int q = 0; q < 4; q++) {
+    // slot-major and diagonal indexes of the SpMV matrices (A, G, H, GT, and
+    // L, U where mode C's sweeps multiply by them), then the format per
+    // matrix and precision
+    for (int q = 0; q < (k.lu_spmv() ? 6 : 4); q++) {
         CM_TRY(build_slots(*ms[q], k.stream));
         if (ms[q]->slots) maxblk = std::max(maxblk, ms[q]->slots->nb);
         CM_TRY(build_dia(*ms[q], k.stream));
@@ -3313,9 +3321,9 @@ int cm_system_spmv_format(cm_system h, int bits, int which, int format)
 int cm_system_dia_info(cm_system h, int which, int *nd, int *offsets, long long *nrem)
 {
     if (!h || !nd || !offsets || !nrem) return fail(CM_ERR_ARG, "null argument");
-    if (which < 0 || which > 3) return fail(CM_ERR_ARG, "matrix index must be 0..3 (A, G, H, GT)");
+    if (which < 0 || which > 5) return fail(CM_ERR_ARG, "matrix index must be 0..5 (A, G, H, GT, L, U)");
     K64 &k = *h->sys.k64;
-    DevCsr<double> *ms[] = {&k.A, &k.G, &k.H, &k.GT};
+    DevCsr<double> *ms[] = {&k.A, &k.G, &k.H, &k.GT, &k.L, &k.U};
     DevCsr<double> &m = *ms[which];
     *nd = 0;
     *nrem = 0;
@@ -3856,8 +3864,9 @@ constexpr double ONESHOT_DIA_REM = 0.3;
 static int oneshot_formats(System &S)
 {
     K64 &k = *S.k64;
-    for (DevCsr<double> *m : {&k.A, &k.G, &k.H, &k.GT}) {
+    for (DevCsr<double> *m : {&k.A, &k.G, &k.H, &k.GT, &k.L, &k.U}) {
         if (!m->present()) continue;
+        if ((m == &k.L || m == &k.U) && !k.lu_spmv()) continue;  // only triangular solves read them
         bool done = false;
         if (!m->dia) CM_TRY(build_dia_device(*m, k.stream, &done));
         if (!m->dia || (double)m->dia->nrem > ONESHOT_DIA_REM * (double)m->nnz) continue;

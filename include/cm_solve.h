@@ -191,7 +191,8 @@ int cm_system_block_time_spmm(cm_system sys, char which, int bits, int reps, dou
 /* Autotuner of the single-RHS SpMV: for each resident matrix (A, G, H,
  * G^T, L, U), times reps FP64 products with row blocks cut for 1024, 2048
  * and 4096 staged nonzeros per workgroup and keeps the fastest (the FP32 /
- * FP16 copies follow).  Then, for A, G, H and G^T in every enabled
+ * FP16 copies follow).  Then, for A, G, H and G^T (and L and U for ACIFIM
+ * apply mode C with nr > 1, whose sweeps multiply by them) in every enabled
  * precision, times each available SpMV format (see cm_system_spmv_format)
  * and keeps the fastest.  tiles[6]: the tile choices, 0 for an absent
  * matrix.  Results of later solves are unchanged up to the summation order
@@ -208,8 +209,8 @@ int cm_system_autotune(cm_system sys, int reps, int *tiles);
  *   4  diagonals held by at least 60% of the rows stored as dense n-vectors
  *      (no column indices), the other entries as a CSR remainder, one row
  *      per thread (matrices of structured grids; FP16 two rows per thread);
- *   5  the same layout with twice the rows per thread and fewer diagonals in
- *      flight (faster on matrices with many diagonals).
+ *   5  the same layout with twice the rows per thread and 9 diagonals per
+ *      chunk (faster on matrices with many diagonals).
  * CM_ERR_ARG when the format is not available for that copy.  The
  * slot-major and diagonal indexes are built on first use (and shared by the
  * precisions). */
@@ -217,7 +218,7 @@ int cm_system_spmv_format(cm_system sys, int bits, int which, int format);
 /* Current formats: formats[18] = {FP64, FP32, FP16} x {A, G, H, G^T, L, U},
  * -1 for an absent matrix or a precision not enabled. */
 int cm_system_spmv_formats(cm_system sys, int *formats);
-/* Diagonal index of one resident matrix (which 0..3 = A, G, H, G^T), built
+/* Diagonal index of one resident matrix (which 0..5 = A, G, H, G^T, L, U), built
  * if absent: *nd stored diagonals (0 when none is held by 60% of the rows),
  * their offsets (column - row, ascending) in offsets[0..*nd) (room for 32
  * needed), and *nrem entries left to the CSR remainder.  No counterpart in

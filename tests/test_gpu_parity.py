@@ -294,6 +294,37 @@ def test_autotuned_tiles_keep_parity():
     assert np.linalg.norm(b - cmamd.csr_mv_host(A, xm)) / np.linalg.norm(b) <= 1e-8
 
 
+def test_autotuned_mode_c_sweeps_keep_parity():
+    """ACIFIM apply mode C with two sweeps multiplies by L and U, so the
+    tuner times their SpMV formats too (the diagonal format, on a stencil's
+    factors); the solve afterwards meets the oracle's trace, iterations and
+    exit, and the FP32-inner refinement reaches the tolerance."""
+    A = cmamd.gen_convdiff(3, 24, 5.0, seed=31)
+    b = cmamd.csr_mv_host(A, cmamd.gen_uniform(32, A.n))
+    F = cmamd.acifim(A, 0.01, 0.1)
+    sys_ = cmamd.System("acifim", A, F, nr=2, mode="C")
+    sys_.enable_mixed(32)
+    sys_.autotune(reps=3)
+    fm = sys_.spmv_formats()
+    assert fm[64]["L"] != "staged" or fm[64]["U"] != "staged", fm
+    x = cmamd.gen_uniform(33, A.n)
+    assert _rel(sys_.apply(x), oracle.apply(F, x, nr=2, mode="C")) <= SPMV_TOL
+    sys_.set_rhs(b)
+    sys_.set_x(None)
+    rv, it, info, tr = sys_.solve(tol=1e-10, nmax=2000, trace_cap=64)
+    xo, rvo, ito, info_o, tro = oracle.bicgstab("acifim", A, b, F, tol=1e-10, nmax=2000, nr=2, mode="C",
+                                                trace_cap=64)
+    m = min(20, len(tr), len(tro))
+    assert _trace_close(tr[:m], tro[:m])
+    assert info == info_o == 0 and abs(it - ito) <= max(1, int(np.ceil(0.02 * ito)))
+    sys_.set_x(None)
+    rv, outer, inner, info = sys_.solve_mixed(32, tol=1e-10, inner_tol=1e-4, nmax=2000)
+    xm = sys_.get_x()
+    sys_.close()
+    assert info == 0
+    assert np.linalg.norm(b - cmamd.csr_mv_host(A, xm)) / np.linalg.norm(b) <= 1e-10
+
+
 def _slot_case(kind):
     import scipy.sparse as sp
     if kind == "afiim":

@@ -138,17 +138,10 @@ static void check_transpose(const Csr &A)
 {
     std::vector<int> tp, tc, pp, pc;
     std::vector<double> tv, pv;
-    host_transpose(A.n, A.ptr.data(), A.col.data(), A.val.data(), tp, tc, tv, 1);
+    host_transpose(A.n, A.ptr.data(), A.col.data(), A.val.data(), tp, tc, tv);
     CHECK(csr_problem(A.n, tp.data(), tc.data()) == nullptr, "transpose is valid CSR");
-    host_transpose(A.n, tp.data(), tc.data(), tv.data(), pp, pc, pv, 1);
+    host_transpose(A.n, tp.data(), tc.data(), tv.data(), pp, pc, pv);
     CHECK(pp == A.ptr && pc == A.col && pv == A.val, "transpose twice is the identity");
-    // the row chunks of several threads give the same arrays as one pass
-    for (int T : {2, 3, 7}) {
-        std::vector<int> qp, qc;
-        std::vector<double> qv;
-        host_transpose(A.n, A.ptr.data(), A.col.data(), A.val.data(), qp, qc, qv, T);
-        CHECK(qp == tp && qc == tc && qv == tv, "threaded transpose = one pass");
-    }
 }
 
 static void check_sym(const Csr &A)
@@ -156,14 +149,8 @@ static void check_sym(const Csr &A)
     if (A.n > 3000) return;
     std::vector<int> fp, fc;
     std::vector<double> fv;
-    host_sym_expand(A.n, A.ptr.data(), A.col.data(), A.val.data(), fp, fc, fv, 1);
+    host_sym_expand(A.n, A.ptr.data(), A.col.data(), A.val.data(), fp, fc, fv);
     CHECK(csr_problem(A.n, fp.data(), fc.data()) == nullptr, "expansion is valid CSR");
-    for (int T : {2, 5}) {
-        std::vector<int> qp, qc;
-        std::vector<double> qv;
-        host_sym_expand(A.n, A.ptr.data(), A.col.data(), A.val.data(), qp, qc, qv, T);
-        CHECK(qp == fp && qc == fc && qv == fv, "threaded expansion = one pass");
-    }
     std::map<std::pair<int, int>, double> full, want;
     for (int i = 0; i < A.n; i++) {
         for (int k = fp[i]; k < fp[i + 1]; k++) {

@@ -9,7 +9,6 @@
 #include <algorithm>
 #include <climits>
 #include <cstdint>
-#include <thread>
 #include <utility>
 #include <vector>
 
@@ -102,153 +101,69 @@ inline void row_bins(int n, const int *ptr, int short_max, std::vector<int> &sh,
     for (int i = 0; i < n; i++) (ptr[i + 1] - ptr[i] > short_max ? lo : sh).push_back(i);
 }
 
-// Threads for the host passes over large matrices: up to 8, one below
-// 2^16 entries.
-inline int host_threads(long long nnz, int nthreads)
-{
-    if (nthreads <= 0) {
-        const unsigned hc = std::thread::hardware_concurrency();
-        nthreads = nnz < (1 << 16) ? 1 : (int)std::min(8u, hc ? hc : 1u);
-    }
-    return std::max(1, std::min(nthreads, 16));
-}
-
-// Runs fn(t) for t = 0..T-1, t = 0 on the calling thread.
-template <class F>
-inline void host_par(int T, F fn)
-{
-    std::vector<std::thread> th;
-    for (int t = 1; t < T; t++) th.emplace_back(fn, t);
-    fn(0);
-    for (auto &x : th) x.join();
-}
-
-// CSR of the transpose of the entries (i, col[k]) that keep(i, col[k])
-// selects, each of its rows in increasing original row (the order one pass
-// over the rows would give).  T threads take contiguous row chunks of about
-// equal entry counts, each with its own column counts, so a column's entries
-// from an earlier chunk come first.
-template <class Keep>
-inline void host_transpose_sel(int n, const int *ptr, const int *col, const double *val, Keep keep,
-                               std::vector<int> &tp, std::vector<int> &tc, std::vector<double> &tv, int T)
-{
-    const long long nnz = ptr[n];
-    std::vector<int> rb((size_t)T + 1, n);
-    rb[0] = 0;
-    for (int t = 1; t < T; t++)
-        rb[t] = (int)(std::lower_bound(ptr, ptr + n + 1, (int)(nnz * t / T)) - ptr);
-    std::vector<std::vector<int>> cnt((size_t)T);
-    host_par(T, [&](int t) {
-        cnt[t].assign((size_t)n, 0);
-        int *c = cnt[t].data();
-        for (int i = rb[t]; i < rb[t + 1]; i++)
-            for (int k = ptr[i]; k < ptr[i + 1]; k++)
-                if (keep(i, col[k])) c[col[k]]++;
-    });
-    tp.assign((size_t)n + 1, 0);
-    auto cols = [&](int t, int &j0, int &j1) {
-        j0 = (int)((long long)n * t / T);
-        j1 = (int)((long long)n * (t + 1) / T);
-    };
-    host_par(T, [&](int t) {  // column totals
-        int j0, j1;
-        cols(t, j0, j1);
-        for (int j = j0; j < j1; j++) {
-            int s = 0;
-            for (int u = 0; u < T; u++) s += cnt[u][j];
-            tp[(size_t)j + 1] = s;
-        }
-    });
-    for (int j = 0; j < n; j++) tp[(size_t)j + 1] += tp[j];
-    host_par(T, [&](int t) {  // each chunk's first position in each column
-        int j0, j1;
-        cols(t, j0, j1);
-        for (int j = j0; j < j1; j++) {
-            int base = tp[j];
-            for (int u = 0; u < T; u++) {
-                const int c = cnt[u][j];
-                cnt[u][j] = base;
-                base += c;
-            }
-        }
-    });
-    tc.resize((size_t)tp[n]);
-    tv.resize((size_t)tp[n]);
-    host_par(T, [&](int t) {
-        int *pos = cnt[t].data();
-        for (int i = rb[t]; i < rb[t + 1]; i++)
-            for (int k = ptr[i]; k < ptr[i + 1]; k++) {
-                const int j = col[k];
-                if (!keep(i, j)) continue;
-                const int q = pos[j]++;
-                tc[q] = i;
-                tv[q] = val[k];
-            }
-    });
-}
-
-// CSR of the transpose (nthreads 0: host_threads' choice).
+// CSR of the transpose.
 inline void host_transpose(int n, const int *ptr, const int *col, const double *val, std::vector<int> &tp,
-                           std::vector<int> &tc, std::vector<double> &tv, int nthreads = 0)
+                           std::vector<int> &tc, std::vector<double> &tv)
 {
-    host_transpose_sel(n, ptr, col, val, [](int, int) { return true; }, tp, tc, tv,
-                       host_threads(ptr[n], nthreads));
+    tp.assign(n + 1, 0);
+    for (int k = 0; k < ptr[n]; k++) tp[col[k] + 1]++;
+    for (int i = 0; i < n; i++) tp[i + 1] += tp[i];
+    tc.resize(ptr[n]);
+    tv.resize(ptr[n]);
+    std::vector<int> fill(tp.begin(), tp.end() - 1);
+    for (int i = 0; i < n; i++)
+        for (int k = ptr[i]; k < ptr[i + 1]; k++) {
+            int q = fill[col[k]]++;
+            tc[q] = i;
+            tv[q] = val[k];
+        }
 }
 
 // Full CSR of a symmetric matrix given its upper triangle (entries below the
 // diagonal, if any, are ignored as the SYMMETRIC/UPPER descriptor does).
-// Row i: the entries below the diagonal (the transpose of the strictly upper
-// part, in increasing column), then the stored upper part of the row.
 inline void host_sym_expand(int n, const int *ptr, const int *col, const double *val, std::vector<int> &fp,
-                            std::vector<int> &fc, std::vector<double> &fv, int nthreads = 0)
+                            std::vector<int> &fc, std::vector<double> &fv)
 {
-    const int T = host_threads(ptr[n], nthreads);
-    std::vector<int> lp, lc;
-    std::vector<double> lv;
-    host_transpose_sel(n, ptr, col, val, [](int i, int j) { return j > i; }, lp, lc, lv, T);
-    fp.assign((size_t)n + 1, 0);
-    auto rows = [&](int t, int &i0, int &i1) {
-        i0 = (int)((long long)n * t / T);
-        i1 = (int)((long long)n * (t + 1) / T);
-    };
-    host_par(T, [&](int t) {
-        int i0, i1;
-        rows(t, i0, i1);
-        for (int i = i0; i < i1; i++) {
-            int c = lp[(size_t)i + 1] - lp[i];
-            for (int k = ptr[i]; k < ptr[i + 1]; k++) c += col[k] >= i;
-            fp[(size_t)i + 1] = c;
+    fp.assign(n + 1, 0);
+    for (int i = 0; i < n; i++)
+        for (int k = ptr[i]; k < ptr[i + 1]; k++) {
+            const int j = col[k];
+            if (j < i) continue;
+            fp[i + 1]++;
+            if (j != i) fp[j + 1]++;
         }
-    });
-    for (int i = 0; i < n; i++) fp[(size_t)i + 1] += fp[i];
-    fc.resize((size_t)fp[n]);
-    fv.resize((size_t)fp[n]);
-    host_par(T, [&](int t) {
-        int i0, i1;
-        rows(t, i0, i1);
-        for (int i = i0; i < i1; i++) {
-            int q = fp[i];
-            for (int k = lp[i]; k < lp[(size_t)i + 1]; k++, q++) {
-                fc[q] = lc[k];
-                fv[q] = lv[k];
-            }
-            for (int k = ptr[i]; k < ptr[i + 1]; k++)
-                if (col[k] >= i) {
-                    fc[q] = col[k];
-                    fv[q] = val[k];
-                    q++;
-                }
+    for (int i = 0; i < n; i++) fp[i + 1] += fp[i];
+    fc.resize(fp[n]);
+    fv.resize(fp[n]);
+    std::vector<int> fill(fp.begin(), fp.end() - 1);
+    // lower part first (column order ascending because rows are scanned in
+    // order), then the stored upper part of the row
+    for (int i = 0; i < n; i++)
+        for (int k = ptr[i]; k < ptr[i + 1]; k++) {
+            const int j = col[k];
+            if (j <= i) continue;
+            int q = fill[j]++;
+            fc[q] = i;
+            fv[q] = val[k];
         }
-    });
+    for (int i = 0; i < n; i++)
+        for (int k = ptr[i]; k < ptr[i + 1]; k++) {
+            const int j = col[k];
+            if (j < i) continue;
+            int q = fill[i]++;
+            fc[q] = j;
+            fv[q] = val[k];
+        }
 }
 
 // Dependency levels of a triangular factor: lower -> a row reads only
 // smaller columns (rows in increasing order); upper -> larger columns.
-// Entries on the other side of the diagonal are ignored.  rows[loff[l] ..
-// loff[l + 1]) are level l's rows.  The declared triangle's entries
-// (diagonal included) are also stored in that row order, tptr / tcol / tsrc
-// (tsrc: the entry's CSR position), so a level's rows read their entries
-// from one contiguous run.
+// Entries on the other side of the diagonal are ignored.  rows lists the
+// rows level by level (loff[l] .. loff[l + 1]).
+// Dependency levels of a triangular factor: rows[loff[l] .. loff[l + 1]) are
+// level l's rows.  The declared triangle's entries (diagonal included) are
+// also stored in that row order, tptr / tcol / tsrc (tsrc: the entry's CSR
+// position), so a level's rows read their entries from one contiguous run.
 struct LevelSchedule {
     int nlev = 0;
     std::vector<int> loff, rows, tptr, tcol, tsrc;

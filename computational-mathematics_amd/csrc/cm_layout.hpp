@@ -158,12 +158,11 @@ inline void host_sym_expand(int n, const int *ptr, const int *col, const double 
 
 // Dependency levels of a triangular factor: lower -> a row reads only
 // smaller columns (rows in increasing order); upper -> larger columns.
-// Entries on the other side of the diagonal are ignored.  rows lists the
-// rows level by level (loff[l] .. loff[l + 1]).
-// Dependency levels of a triangular factor: rows[loff[l] .. loff[l + 1]) are
-// level l's rows.  The declared triangle's entries (diagonal included) are
-// also stored in that row order, tptr / tcol / tsrc (tsrc: the entry's CSR
-// position), so a level's rows read their entries from one contiguous run.
+// Entries on the other side of the diagonal are ignored.  rows[loff[l] ..
+// loff[l + 1]) are level l's rows.  The declared triangle's entries
+// (diagonal included) are also stored in that row order, tptr / tcol / tsrc
+// (tsrc: the entry's CSR position), so a level's rows read their entries
+// from one contiguous run.
 struct LevelSchedule {
     int nlev = 0;
     std::vector<int> loff, rows, tptr, tcol, tsrc;

@@ -656,7 +656,7 @@ def main():
     ap.add_argument("--serve-rate", type=float, default=200.0, help="Poisson arrival rate (requests/s)")
     ap.add_argument("--serve-slots", type=int, default=4)
     ap.add_argument("--serve-seed", type=int, default=5)
-    ap.add_argument("--serve-small-n", type=int, default=8192,
+    ap.add_argument("--serve-small-n", type=int, default=4096,
                     help="systems with at most this many rows go to the persistent batched kernel (0: never)")
     ap.add_argument("--no-autotune", action="store_true", help="keep the default SpMV row blocks and kernels")
     ap.add_argument("--no-graphs", action="store_true",
